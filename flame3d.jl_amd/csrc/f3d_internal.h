@@ -1,0 +1,271 @@
+// Internal declarations shared by the libflame3d translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/flame3d.h"
+
+namespace f3d {
+
+constexpr int MAXNS = 16;
+constexpr int MAXNR = 48;
+constexpr int NG = 4;
+constexpr double kEps = 2.220446049250313e-16;  // eps(Float64)
+
+// Parameter block resident in device memory; kernels read it through a
+// uniform pointer, so the loads are scalar.  Arrays are transposed to
+// [species][coefficient] so one species' coefficients are contiguous.
+struct DevParams {
+  int Ns, Nr;
+  double Ru, min_temp, max_temp, atm;
+  double mw[MAXNS], sep[MAXNS];
+  double lo[MAXNS][7], hi[MAXNS][7];
+  double visc[MAXNS][5], cond[MAXNS][5];
+  double bdiff[MAXNS][MAXNS][5];  // [n][nn][c] = binarydiff_poly[c, nn, n]
+  int rtype[MAXNR], sgm[MAXNR];
+  int vf[MAXNR][MAXNS], vr[MAXNR][MAXNS];
+  double Arr[MAXNR][3], ef[MAXNR][MAXNS], loP[MAXNR][3], Troe[MAXNR][4];
+  double SWeps2, WENOeps, phi1, phi2, UP7[7], T_criteria;
+  // float copies for the FP32 flux path
+  float mw_f[MAXNS], sep_f[MAXNS], lo_f[MAXNS][7], hi_f[MAXNS][7];
+  float visc_f[MAXNS][5], cond_f[MAXNS][5], bdiff_f[MAXNS][MAXNS][5];
+  float Ru_f, WENOeps_f, phi1_f, phi2_f, SWeps2_f, UP7_f[7];
+};
+
+// Padded-block geometry (0-based).  Point p = i + Nxt*(j + Nyt*k); variable
+// n of a padded array lives at p + n*NP.
+struct Geom {
+  int Nx, Ny, Nz, Ns, Nr;
+  int Nxt, Nyt, Nzt;
+  int NP;
+};
+
+}  // namespace f3d
+
+struct f3d_ctx {
+  f3d::Geom g;
+  int precision;
+  int bug_compat;
+  f3d::DevParams host_params;
+  f3d::DevParams* dparams = nullptr;
+  // boundary fills
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  double inflow_Q[7] = {0, 0, 0, 0, 0, 0, 0};
+  double inflow_rhoi[f3d::MAXNS] = {0};
+  // metrics registered by f3d_set_metrics
+  const double* metrics[9] = {nullptr};
+  const double* J = nullptr;
+  // context-owned workspace for f3d_rk3_stage
+  void* ws_base = nullptr;
+  size_t ws_bytes = 0;
+  void* ws[32] = {nullptr};
+};
+
+namespace f3d {
+
+void set_error(const std::string& msg);
+int hip_check(hipError_t e, const char* what);
+
+// ---- launch helpers ---------------------------------------------------------
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+// Bijective remap of blockIdx.x so that consecutive logical blocks run on the
+// same XCD (workgroups are dispatched round-robin over the 8 XCDs, each with
+// its own L2): neighbouring tiles then share an L2.
+__device__ inline unsigned xcd_block() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+  const unsigned base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return base + k;
+}
+
+__device__ inline long gid() { return (long)xcd_block() * blockDim.x + threadIdx.x; }
+
+// ---- typed parameter access -----------------------------------------------
+template <typename R>
+struct PT;
+template <>
+struct PT<double> {
+  __device__ static const double* lo(const DevParams* P, int n) { return P->lo[n]; }
+  __device__ static const double* hi(const DevParams* P, int n) { return P->hi[n]; }
+  __device__ static double sep(const DevParams* P, int n) { return P->sep[n]; }
+  __device__ static double mw(const DevParams* P, int n) { return P->mw[n]; }
+  __device__ static const double* visc(const DevParams* P, int n) { return P->visc[n]; }
+  __device__ static const double* cond(const DevParams* P, int n) { return P->cond[n]; }
+  __device__ static const double* bdiff(const DevParams* P, int n, int nn) { return P->bdiff[n][nn]; }
+  __device__ static double Ru(const DevParams* P) { return P->Ru; }
+  __device__ static double weps(const DevParams* P) { return P->WENOeps; }
+  __device__ static double phi1(const DevParams* P) { return P->phi1; }
+  __device__ static double phi2(const DevParams* P) { return P->phi2; }
+  __device__ static double sweps2(const DevParams* P) { return P->SWeps2; }
+  __device__ static const double* up7(const DevParams* P) { return P->UP7; }
+  __device__ static double eps() { return kEps; }
+};
+template <>
+struct PT<float> {
+  __device__ static const float* lo(const DevParams* P, int n) { return P->lo_f[n]; }
+  __device__ static const float* hi(const DevParams* P, int n) { return P->hi_f[n]; }
+  __device__ static float sep(const DevParams* P, int n) { return P->sep_f[n]; }
+  __device__ static float mw(const DevParams* P, int n) { return P->mw_f[n]; }
+  __device__ static const float* visc(const DevParams* P, int n) { return P->visc_f[n]; }
+  __device__ static const float* cond(const DevParams* P, int n) { return P->cond_f[n]; }
+  __device__ static const float* bdiff(const DevParams* P, int n, int nn) { return P->bdiff_f[n][nn]; }
+  __device__ static float Ru(const DevParams* P) { return P->Ru_f; }
+  __device__ static float weps(const DevParams* P) { return P->WENOeps_f; }
+  __device__ static float phi1(const DevParams* P) { return P->phi1_f; }
+  __device__ static float phi2(const DevParams* P) { return P->phi2_f; }
+  __device__ static float sweps2(const DevParams* P) { return P->SWeps2_f; }
+  __device__ static const float* up7(const DevParams* P) { return P->UP7_f; }
+  // eps(Float64) is used as a floor in the reference; kept as the same value
+  __device__ static float eps() { return (float)kEps; }
+};
+
+// ---- thermo device functions (thermo.jl) -----------------------------------
+
+// h_specs, thermo.jl:107-133 (J/kg)
+template <int NS, typename R>
+__device__ inline void h_specs(const DevParams* P, R T, R* h) {
+  const R T2 = R(0.5) * T * T;
+  const R T3 = R(2.0 / 3.0) * T2 * T;
+  const R T4 = T2 * T2;
+  const R T5 = R(0.8) * T4 * T;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const R* lo = PT<R>::lo(P, n);
+    const R* hc = PT<R>::hi(P, n);
+    R v;
+    if (T < PT<R>::sep(P, n)) {
+      v = lo[0] * T + lo[1] * T2 + lo[2] * T3 + lo[3] * T4 + lo[4] * T5;
+    } else {
+      v = hc[0] * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 + (hc[5] - lo[5]);
+    }
+    h[n] = v * (PT<R>::Ru(P) / PT<R>::mw(P, n));
+  }
+}
+
+// ei_specs, thermo.jl:136-160 and InternalEnergy, thermo.jl:219-228 (FP64)
+template <int NS>
+__device__ inline double internal_energy(const DevParams* P, double T, const double* rhoi) {
+  const double T2 = 0.5 * T * T;
+  const double T3 = 2.0 / 3.0 * T2 * T;
+  const double T4 = T2 * T2;
+  const double T5 = 0.8 * T4 * T;
+  double result = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const double* lo = P->lo[n];
+    const double* hc = P->hi[n];
+    double e;
+    if (T < P->sep[n]) {
+      e = (lo[0] - 1) * T + lo[1] * T2 + lo[2] * T3 + lo[3] * T4 + lo[4] * T5;
+    } else {
+      e = (hc[0] - 1) * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 + (hc[5] - lo[5]);
+    }
+    result += rhoi[n] * e / P->mw[n];
+  }
+  return result * P->Ru;
+}
+
+// CV, thermo.jl:194-203 (FP64)
+template <int NS>
+__device__ inline double cv_mix(const DevParams* P, double T, const double* rhoi) {
+  const double T2 = T * T;
+  const double T3 = T2 * T;
+  const double T4 = T2 * T2;
+  double result = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const double* c = (T < P->sep[n]) ? P->lo[n] : P->hi[n];
+    const double cp = c[0] + c[1] * T + c[2] * T2 + c[3] * T3 + c[4] * T4;
+    result += (cp - 1) * rhoi[n] / P->mw[n];
+  }
+  return result * P->Ru;
+}
+
+// Pmixture, thermo.jl:20-26
+template <int NS>
+__device__ inline double pmixture(const DevParams* P, double T, const double* rhoi) {
+  double YOW = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) YOW += rhoi[n] / P->mw[n];
+  return P->Ru * T * YOW;
+}
+
+// GetT, thermo.jl:231-282 (FP64 Newton e -> T; tol 1e-3 K, <= 30 updates,
+// +-100 K step clamp, linear extrapolation outside [min_temp, max_temp]).
+template <int NS>
+__device__ inline double get_T(const DevParams* P, double ein, const double* rhoi, int* iters) {
+  const double tol = 1e-3;
+  const double tmin = P->min_temp, tmax = P->max_temp;
+  const double emin = internal_energy<NS>(P, tmin, rhoi);
+  const double emax = internal_energy<NS>(P, tmax, rhoi);
+  if (iters) *iters = 0;
+  if (ein < emin) return tmin - (emin - ein) / cv_mix<NS>(P, tmin, rhoi);
+  if (ein > emax) return tmax - (emax - ein) / cv_mix<NS>(P, tmax, rhoi);
+  double As = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) As += (P->lo[n][0] - 1) * P->Ru / P->mw[n] * rhoi[n];
+  double t1 = ein / As;
+  if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
+  int it = 0;
+  for (int m = 0; m < 30; ++m) {
+    const double e1 = internal_energy<NS>(P, t1, rhoi);
+    const double cv = cv_mix<NS>(P, t1, rhoi);
+    double dt = (ein - e1) / cv;
+    if (dt > 100.0) {
+      dt = 100.0;
+    } else if (dt < -100.0) {
+      dt = -100.0;
+    } else if (fabs(dt) < tol) {
+      break;
+    } else if (t1 + dt == t1) {
+      break;
+    }
+    t1 += dt;
+    ++it;
+  }
+  if (iters) *iters = it;
+  return t1;
+}
+
+// ---- internal launchers (defined in the .hip files) ------------------------
+int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
+                      const double* Ay, const double* Az, const int64_t* tag, hipStream_t s);
+int launch_species_split(f3d_ctx* c, const double* rhoi, const double* Q, void* Fp, void* Fm,
+                         const double* Ax, const double* Ay, const double* Az,
+                         const int64_t* tag, hipStream_t s);
+int launch_shock_sensor(f3d_ctx* c, void* phi, const double* Q, hipStream_t s);
+int launch_weno(f3d_ctx* c, int dir, void* F, const void* phi, const void* Fp, const void* Fm,
+                int NV, const int64_t* tag, hipStream_t s);
+int launch_viscous(f3d_ctx* c, int dir, void* Fv, const double* Q, const double* const* M,
+                   const double* J, const void* lam, const void* mu, const void* Fh,
+                   const int64_t* tag, hipStream_t s);
+int launch_spec_viscous(f3d_ctx* c, int dir, void* Fd, const double* Q, const void* Yi,
+                        const double* const* M, const double* J, const void* D, void* Fh,
+                        const int64_t* tag, hipStream_t s);
+int launch_div(f3d_ctx* c, double* U, const void* Fx, const void* Fy, const void* Fz,
+               const void* Fvx, const void* Fvy, const void* Fvz, double dt, const double* J,
+               const int64_t* tag, hipStream_t s);
+int launch_div_specs(f3d_ctx* c, double* rhoi, const void* Fx, const void* Fy, const void* Fz,
+                     const void* Fdx, const void* Fdy, const void* Fdz, double dt,
+                     const double* J, const int64_t* tag, hipStream_t s);
+int launch_mixture(f3d_ctx* c, const double* Q, const double* rhoi, void* Yi, void* lam,
+                   void* mu, void* D, const int64_t* tag, hipStream_t s);
+int launch_c2prim(f3d_ctx* c, double* U, double* Q, double* rhoi, const int64_t* tag,
+                  hipStream_t s);
+int launch_prim2c(f3d_ctx* c, double* U, const double* Q, hipStream_t s);
+int launch_lincomb(f3d_ctx* c, double* U, const double* Un, int NV, double a, double b,
+                   const int64_t* tag, hipStream_t s);
+int launch_get_y(f3d_ctx* c, double* Yi, const double* rhoi, const double* Q,
+                 const int64_t* tag, hipStream_t s);
+int launch_fill(f3d_ctx* c, double* Q, double* U, double* rhoi, hipStream_t s);
+int launch_chem(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,
+                int maxiter, double rtol, double ttol, int32_t* iters, const int64_t* tag,
+                hipStream_t s);
+
+}  // namespace f3d

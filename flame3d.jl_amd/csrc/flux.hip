@@ -1,0 +1,255 @@
+// Convective path: Steger-Warming splitting (split.jl), pressure shock sensor
+// and hybrid UP7 / WENO5-Z / NND face reconstruction (schemes.jl).
+//
+// One thread per padded point (splitting, sensor) or per face
+// (reconstruction), 1-D grids in i-fastest order so every wavefront reads 64
+// consecutive i of one variable plane (coalesced), with XCD-aware block
+// ordering so y/z stencil neighbours are reused from the same L2.
+// Template parameter R is the arithmetic/storage type of the intermediates
+// (double for the FP64 path, float for the FP32 path); the state Q, rho_k
+// and the metrics are read as double.
+#include "f3d_internal.h"
+
+namespace f3d {
+
+// fluxSplit, split.jl:17-89
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_flux_split(const double* __restrict__ Q, R* __restrict__ Fp, R* __restrict__ Fm,
+                 const double* __restrict__ Ax, const double* __restrict__ Ay,
+                 const double* __restrict__ Az, const int64_t* __restrict__ tag,
+                 const DevParams* __restrict__ P, int NP) {
+  const long p = gid();
+  if (p >= NP) return;
+  if (tag && tag[p] == 1) return;
+  const R rho = (R)Q[p], u = (R)Q[p + NP], v = (R)Q[p + 2L * NP], w = (R)Q[p + 3L * NP];
+  const R pr = (R)Q[p + 4L * NP], ei = (R)Q[p + 6L * NP];
+  R A1 = (R)Ax[p], A2 = (R)Ay[p], A3 = (R)Az[p];
+  const R SWeps2 = PT<R>::sweps2(P);
+  const R gam = pr / ei + R(1);
+  const R c = sqrt(gam * pr / rho);
+  R ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
+  const R E1 = A1 * u + A2 * v + A3 * w;
+  const R E2 = E1 - c * ss;
+  const R E3 = E1 + c * ss;
+  ss = R(1) / ss;
+  A1 *= ss;
+  A2 *= ss;
+  A3 *= ss;
+  const R E1P = (E1 + sqrt(E1 * E1 + SWeps2)) * R(0.5);
+  const R E2P = (E2 + sqrt(E2 * E2 + SWeps2)) * R(0.5);
+  const R E3P = (E3 + sqrt(E3 * E3 + SWeps2)) * R(0.5);
+  const R E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+  const R uc1 = u - c * A1, uc2 = u + c * A1;
+  const R vc1 = v - c * A2, vc2 = v + c * A2;
+  const R wc1 = w - c * A3, wc2 = w + c * A3;
+  const R vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * R(0.5);
+  const R vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * R(0.5);
+  const R vv = (gam - R(1)) * (u * u + v * v + w * w);
+  const R W2 = (R(3) - gam) / (R(2) * (gam - R(1))) * (c * c);
+  const R tmp1 = rho / (R(2) * gam);
+  const R tmp2 = R(2) * (gam - R(1));
+  Fp[p] = tmp1 * (tmp2 * E1P + E2P + E3P);
+  Fp[p + NP] = tmp1 * (tmp2 * E1P * u + E2P * uc1 + E3P * uc2);
+  Fp[p + 2L * NP] = tmp1 * (tmp2 * E1P * v + E2P * vc1 + E3P * vc2);
+  Fp[p + 3L * NP] = tmp1 * (tmp2 * E1P * w + E2P * wc1 + E3P * wc2);
+  Fp[p + 4L * NP] = tmp1 * (E1P * vv + E2P * vvc1 + E3P * vvc2 + W2 * (E2P + E3P));
+  Fm[p] = tmp1 * (tmp2 * E1M + E2M + E3M);
+  Fm[p + NP] = tmp1 * (tmp2 * E1M * u + E2M * uc1 + E3M * uc2);
+  Fm[p + 2L * NP] = tmp1 * (tmp2 * E1M * v + E2M * vc1 + E3M * vc2);
+  Fm[p + 3L * NP] = tmp1 * (tmp2 * E1M * w + E2M * wc1 + E3M * wc2);
+  Fm[p + 4L * NP] = tmp1 * (E1M * vv + E2M * vvc1 + E3M * vvc2 + W2 * (E2M + E3M));
+}
+
+// split, split.jl:101-130
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_species_split(const double* __restrict__ rhoi, const double* __restrict__ Q,
+                    R* __restrict__ Fp, R* __restrict__ Fm, const double* __restrict__ Ax,
+                    const double* __restrict__ Ay, const double* __restrict__ Az,
+                    const int64_t* __restrict__ tag, int NP, int Ns) {
+  const long p = gid();
+  if (p >= NP) return;
+  if (tag && tag[p] == 1) return;
+  const R u = (R)Q[p + NP], v = (R)Q[p + 2L * NP], w = (R)Q[p + 3L * NP];
+  const R un = (R)Ax[p] * u + (R)Ay[p] * v + (R)Az[p] * w;
+  const R Ep = R(0.5) * (un + fabs(un));
+  const R Em = R(0.5) * (un - fabs(un));
+  for (int n = 0; n < Ns; ++n) {
+    const R r = (R)rhoi[p + (long)n * NP];
+    Fp[p + (long)n * NP] = Ep * r;
+    Fm[p + (long)n * NP] = Em * r;
+  }
+}
+
+// shockSensor, schemes.jl:1-24 (0-based range [1, N+2NG-2] per direction)
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_shock_sensor(R* __restrict__ phi, const double* __restrict__ Q, Geom g) {
+  const long p = gid();
+  if (p >= g.NP) return;
+  const int i = (int)(p % g.Nxt);
+  const int t = (int)(p / g.Nxt);
+  const int j = t % g.Nyt, k = t / g.Nyt;
+  if (i < 1 || i > g.Nxt - 2 || j < 1 || j > g.Nyt - 2 || k < 1 || k > g.Nzt - 2) return;
+  const double* Pp = Q + 4L * g.NP;
+  const long sy = g.Nxt, sz = (long)g.Nxt * g.Nyt;
+  const R Px1 = (R)Pp[p - 1], Px2 = (R)Pp[p], Px3 = (R)Pp[p + 1];
+  const R Py1 = (R)Pp[p - sy], Py3 = (R)Pp[p + sy];
+  const R Pz1 = (R)Pp[p - sz], Pz3 = (R)Pp[p + sz];
+  const R px = fabs(-Px1 + R(2) * Px2 - Px3) / (Px1 + R(2) * Px2 + Px3);
+  const R py = fabs(-Py1 + R(2) * Px2 - Py3) / (Py1 + R(2) * Px2 + Py3);
+  const R pz = fabs(-Pz1 + R(2) * Px2 - Pz3) / (Pz1 + R(2) * Px2 + Pz3);
+  phi[p] = px + py + pz;
+}
+
+template <typename R>
+__device__ inline R minmod(R a, R b) {
+  // schemes.jl:26-28
+  return (a * b > R(0)) ? ((fabs(a) > fabs(b)) ? b : a) : R(0);
+}
+
+// WENO_x / WENO_y / WENO_z face value, schemes.jl:86-437.  fp/fm point at the
+// variable plane, ic is the face's right cell, s the sweep stride.
+template <typename R>
+__device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm, long ic, long s,
+                              R phif, const DevParams* __restrict__ P) {
+  const R* c = PT<R>::up7(P);
+  if (phif < PT<R>::phi1(P)) {
+    const R fpv = c[0] * fp[ic - 4 * s] + c[1] * fp[ic - 3 * s] + c[2] * fp[ic - 2 * s] +
+                  c[3] * fp[ic - s] + c[4] * fp[ic] + c[5] * fp[ic + s] + c[6] * fp[ic + 2 * s];
+    const R fmv = c[6] * fm[ic - 3 * s] + c[5] * fm[ic - 2 * s] + c[4] * fm[ic - s] +
+                  c[3] * fm[ic] + c[2] * fm[ic + s] + c[1] * fm[ic + 2 * s] + c[0] * fm[ic + 3 * s];
+    return fpv + fmv;
+  } else if (phif < PT<R>::phi2(P)) {
+    const R tmp1 = R(13.0 / 12.0), tmp2 = R(1.0 / 6.0);
+    const R e = PT<R>::weps(P);
+    R V1 = fp[ic - 3 * s], V2 = fp[ic - 2 * s], V3 = fp[ic - s], V4 = fp[ic], V5 = fp[ic + s];
+    R t = V1 - R(2) * V2 + V3, t2 = V1 - R(4) * V2 + R(3) * V3;
+    R s11 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    t = V2 - R(2) * V3 + V4;
+    t2 = V2 - V4;
+    R s22 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    t = V3 - R(2) * V4 + V5;
+    t2 = R(3) * V3 - R(4) * V4 + V5;
+    R s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    R tau = fabs(s11 - s33);
+    t = tau / (e + s11);
+    s11 = (R(1) + t * t) * R(1);
+    t = tau / (e + s22);
+    s22 = (R(1) + t * t) * R(6);
+    t = tau / (e + s33);
+    s33 = (R(1) + t * t) * R(3);
+    R invsum = R(1) / (s11 + s22 + s33);
+    R v1 = R(2) * V1 - R(7) * V2 + R(11) * V3;
+    R v2 = -V2 + R(5) * V3 + R(2) * V4;
+    R v3 = R(2) * V3 + R(5) * V4 - V5;
+    const R fpv = tmp2 * invsum * (s11 * v1 + s22 * v2 + s33 * v3);
+
+    V1 = fm[ic - 2 * s];
+    V2 = fm[ic - s];
+    V3 = fm[ic];
+    V4 = fm[ic + s];
+    V5 = fm[ic + 2 * s];
+    t = V5 - R(2) * V4 + V3;
+    t2 = V5 - R(4) * V4 + R(3) * V3;
+    s11 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    t = V2 - R(2) * V3 + V4;
+    t2 = V4 - V2;
+    s22 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    t = V3 - R(2) * V2 + V1;
+    t2 = R(3) * V3 - R(4) * V2 + V1;
+    s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
+    tau = fabs(s11 - s33);
+    t = tau / (e + s11);
+    s11 = (R(1) + t * t) * R(1);
+    t = tau / (e + s22);
+    s22 = (R(1) + t * t) * R(6);
+    t = tau / (e + s33);
+    s33 = (R(1) + t * t) * R(3);
+    invsum = R(1) / (s11 + s22 + s33);
+    v1 = R(11) * V3 - R(7) * V4 + R(2) * V5;
+    v2 = -V4 + R(5) * V3 + R(2) * V2;
+    v3 = R(2) * V3 + R(5) * V2 - V1;
+    const R fmv = tmp2 * invsum * (s11 * v1 + s22 * v2 + s33 * v3);
+    return fpv + fmv;
+  } else {
+    const R fpv = fp[ic - s] + R(0.5) * minmod(fp[ic] - fp[ic - s], fp[ic - s] - fp[ic - 2 * s]);
+    const R fmv = fm[ic] - R(0.5) * minmod(fm[ic + s] - fm[ic], fm[ic] - fm[ic - s]);
+    return fpv + fmv;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_weno(R* __restrict__ F, const R* __restrict__ phi, const R* __restrict__ Fp,
+           const R* __restrict__ Fm, int NV, const int64_t* __restrict__ tag,
+           const DevParams* __restrict__ P, Geom g, int dir) {
+  const int nx = g.Nx + (dir == 0), ny = g.Ny + (dir == 1), nz = g.Nz + (dir == 2);
+  const long nf = (long)nx * ny * nz;
+  const long f = gid();
+  if (f >= nf) return;
+  const int a = (int)(f % nx);
+  const int t = (int)(f / nx);
+  const int b = t % ny, c = t / ny;
+  const long ic = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  if (tag && tag[ic] == 1) return;
+  const long s = (dir == 0) ? 1 : (dir == 1) ? (long)g.Nxt : (long)g.Nxt * g.Nyt;
+  const R phif = fmax(phi[ic - s], phi[ic]);
+  for (int n = 0; n < NV; ++n)
+    F[f + n * nf] = weno_face<R>(Fp + (long)n * g.NP, Fm + (long)n * g.NP, ic, s, phif, P);
+}
+
+// ---- launchers --------------------------------------------------------------
+int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
+                      const double* Ay, const double* Az, const int64_t* tag, hipStream_t s) {
+  const int NP = c->g.NP;
+  if (c->precision == F3D_FP64)
+    hipLaunchKernelGGL(k_flux_split<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s, Q,
+                       (double*)Fp, (double*)Fm, Ax, Ay, Az, tag, c->dparams, NP);
+  else
+    hipLaunchKernelGGL(k_flux_split<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s, Q,
+                       (float*)Fp, (float*)Fm, Ax, Ay, Az, tag, c->dparams, NP);
+  return hip_check(hipGetLastError(), "flux_split");
+}
+
+int launch_species_split(f3d_ctx* c, const double* rhoi, const double* Q, void* Fp, void* Fm,
+                         const double* Ax, const double* Ay, const double* Az,
+                         const int64_t* tag, hipStream_t s) {
+  const int NP = c->g.NP;
+  if (c->precision == F3D_FP64)
+    hipLaunchKernelGGL(k_species_split<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s, rhoi, Q,
+                       (double*)Fp, (double*)Fm, Ax, Ay, Az, tag, NP, c->g.Ns);
+  else
+    hipLaunchKernelGGL(k_species_split<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s, rhoi, Q,
+                       (float*)Fp, (float*)Fm, Ax, Ay, Az, tag, NP, c->g.Ns);
+  return hip_check(hipGetLastError(), "species_split");
+}
+
+int launch_shock_sensor(f3d_ctx* c, void* phi, const double* Q, hipStream_t s) {
+  const int NP = c->g.NP;
+  if (c->precision == F3D_FP64)
+    hipLaunchKernelGGL(k_shock_sensor<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s,
+                       (double*)phi, Q, c->g);
+  else
+    hipLaunchKernelGGL(k_shock_sensor<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s,
+                       (float*)phi, Q, c->g);
+  return hip_check(hipGetLastError(), "shock_sensor");
+}
+
+int launch_weno(f3d_ctx* c, int dir, void* F, const void* phi, const void* Fp, const void* Fm,
+                int NV, const int64_t* tag, hipStream_t s) {
+  const Geom& g = c->g;
+  const long nf = (long)(g.Nx + (dir == 0)) * (g.Ny + (dir == 1)) * (g.Nz + (dir == 2));
+  if (c->precision == F3D_FP64)
+    hipLaunchKernelGGL(k_weno<double>, dim3(nblocks(nf)), dim3(kBlock), 0, s, (double*)F,
+                       (const double*)phi, (const double*)Fp, (const double*)Fm, NV, tag,
+                       c->dparams, g, dir);
+  else
+    hipLaunchKernelGGL(k_weno<float>, dim3(nblocks(nf)), dim3(kBlock), 0, s, (float*)F,
+                       (const float*)phi, (const float*)Fp, (const float*)Fm, NV, tag,
+                       c->dparams, g, dir);
+  return hip_check(hipGetLastError(), "weno");
+}
+
+}  // namespace f3d

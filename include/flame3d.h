@@ -1,0 +1,193 @@
+/* flame3d.h -- C-ABI of libflame3d.so, the MI355X (gfx950) per-RK-stage
+ * right-hand side of Flame3D.jl.
+ *
+ * Every entry point replaces one `@cuda` launch (or one host function) of the
+ * reference's Julia driver, with the same array arguments in the same layout:
+ * Julia column-major [i, j, k, var] with NG = 4 ghost layers (i fastest,
+ * variable slowest), Float64 state arrays owned by the caller on the device.
+ * Face arrays are [(Nx+1), Ny, Nz, NV], [Nx, (Ny+1), Nz, NV], [Nx, Ny, (Nz+1), NV].
+ * Intermediate arrays (split fluxes, face fluxes, sensor, transport) use the
+ * context's flux precision: double for F3D_FP64, float for F3D_FP32 - they
+ * are passed as void*.  The state U, Q, rhoi, metrics and J are always double.
+ *
+ * All functions return 0 on success and a negative F3D_ERR_* code on failure
+ * (f3d_last_error() gives the message); nothing throws across the ABI.
+ * Launches are asynchronous on the given HIP stream (0 = default stream),
+ * in stream order, exactly as the reference's launches are on its stream.
+ * `tag` may be NULL (no immersed boundary: every point is fluid); when given
+ * it is the reference's Int64 IBM tag array and kernels skip solid points
+ * as the reference does.
+ *
+ * A Julia caller binds these with ccall.
+ */
+#ifndef FLAME3D_H
+#define FLAME3D_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define F3D_OK 0
+#define F3D_ERR_ARG (-1)
+#define F3D_ERR_HIP (-2)
+#define F3D_ERR_ALLOC (-3)
+#define F3D_ERR_UNSUPPORTED (-4)
+#define F3D_ERR_STATE (-5)
+
+#define F3D_FP64 0
+#define F3D_FP32 1
+
+/* fill kinds per face (f3d_set_boundary) */
+#define F3D_BC_KEEP 0     /* ghosts untouched (fixed Dirichlet state) */
+#define F3D_BC_PERIODIC 1 /* ghost <- opposite interior */
+#define F3D_BC_COPY 2     /* zero gradient, boundary node included (boundary.jl:59-84) */
+#define F3D_BC_INFLOW 3   /* fixed uniform state, boundary node included (boundary.jl:1-43) */
+
+typedef struct f3d_ctx f3d_ctx;
+
+/* Block geometry: Nx (= Nxp, this GPU's share), Ny, Nz interior points,
+ * NG ghost layers (must be 4, parse_mesh.jl:5), Ns species, Nr reactions
+ * (run.jl:13-14, :59-63). */
+typedef struct {
+  int32_t Nx, Ny, Nz, NG, Ns, Nr;
+} f3d_dims;
+
+/* thermoProperty (run.jl:71-82) as initThermo fills it (thermo.jl:405-436):
+ * host pointers, column-major. */
+typedef struct {
+  double Ru, min_temp, max_temp;
+  const double* mw;              /* [Ns] kg/mol */
+  const double* coeffs_sep;      /* [Ns] NASA-7 mid temperature */
+  const double* coeffs_lo;       /* [7, Ns] */
+  const double* coeffs_hi;       /* [7, Ns] */
+  const double* visc_poly;       /* [5, Ns] */
+  const double* conduct_poly;    /* [5, Ns] */
+  const double* binarydiff_poly; /* [5, Ns, Ns] */
+} f3d_thermo;
+
+/* reactionProperty (run.jl:84-94) as initReact fills it (reactions.jl:506-576). */
+typedef struct {
+  double atm;
+  const int64_t* reaction_type; /* [Nr] 1 Arrhenius, 2 three-body, 3 Troe */
+  const int64_t* sgm;           /* [Nr] */
+  const int64_t* vf;            /* [Ns, Nr] */
+  const int64_t* vr;            /* [Ns, Nr] */
+  const double* Arr;            /* [3, Nr] A, b, Ea/R */
+  const double* ef;             /* [Ns, Nr] */
+  const double* loP;            /* [3, Nr] */
+  const double* Troe;           /* [4, Nr] */
+} f3d_kinetics;
+
+/* Scheme constants (run.jl:12, :51-57) plus the choices this library adds. */
+typedef struct {
+  double SWeps2, WENOeps, hybrid_phi1, hybrid_phi2;
+  double UP7[7];
+  double T_criteria;
+  int32_t precision;   /* F3D_FP64 or F3D_FP32 (flux/viscous/divergence arithmetic) */
+  int32_t bug_compat;  /* bit 0: eval_gpu_stiff's transposed coeffs_lo[n,6] read
+                          (reactions.jl:282), valid for Ns >= 6 */
+  int32_t reserved[6];
+} f3d_scheme;
+
+/* ---- context ------------------------------------------------------------ */
+/* Copies the parameter bundles to the device.  Replaces the Adapt-ed structs
+ * of run.jl:96-97 and the compile-time constants of run.jl:6-69. */
+int f3d_create(const f3d_dims* dims, const f3d_thermo* thermo, const f3d_kinetics* kin,
+               const f3d_scheme* scheme, f3d_ctx** out);
+int f3d_destroy(f3d_ctx* ctx);
+const char* f3d_last_error(void);
+/* precision of the context's intermediate arrays (sizeof element: 8 or 4) */
+int f3d_real_size(const f3d_ctx* ctx);
+
+/* ---- kernel-granular entry points (one per @cuda line of solver.jl) ----- */
+/* fluxSplit, split.jl:17-89: Steger-Warming F+/F- at every padded point.
+ * Q [..,7], Fp/Fm [..,5]; Ax/Ay/Az the three metrics of the sweep. */
+int f3d_flux_split(f3d_ctx* ctx, const double* Q, void* Fp, void* Fm, const double* Ax,
+                   const double* Ay, const double* Az, const int64_t* tag, hipStream_t s);
+/* split, split.jl:101-130: species F+/F- = 1/2(Un +- |Un|) rho_k. */
+int f3d_species_split(f3d_ctx* ctx, const double* rhoi, const double* Q, void* Fp, void* Fm,
+                      const double* Ax, const double* Ay, const double* Az, const int64_t* tag,
+                      hipStream_t s);
+/* shockSensor, schemes.jl:1-24: phi [padded] from Q[.., 5]. */
+int f3d_shock_sensor(f3d_ctx* ctx, void* phi, const double* Q, hipStream_t s);
+/* WENO_x / WENO_y / WENO_z, schemes.jl:86-437 (dir = 0, 1, 2): hybrid
+ * UP7 / WENO5-Z / NND face flux; NV = 5 (flow) or Ns (species). */
+int f3d_weno(f3d_ctx* ctx, int32_t dir, void* F, const void* phi, const void* Fp, const void* Fm,
+             int32_t NV, const int64_t* tag, hipStream_t s);
+/* viscousFlux_x/y/z, viscous.jl:1-428.  metrics[9] = dxi/dx, dxi/dy, dxi/dz,
+ * deta/dx, deta/dy, deta/dz, dzeta/dx, dzeta/dy, dzeta/dz (device pointers);
+ * lambda, mu [padded]; Fh [faces, 3] from f3d_spec_viscous_flux. */
+int f3d_viscous_flux(f3d_ctx* ctx, int32_t dir, void* Fv, const double* Q,
+                     const double* const* metrics, const double* J, const void* lambda,
+                     const void* mu, const void* Fh, const int64_t* tag, hipStream_t s);
+/* specViscousFlux_x/y/z, viscous.jl:430-665: Fd [faces, Ns], Fh [faces, 3]. */
+int f3d_spec_viscous_flux(f3d_ctx* ctx, int32_t dir, void* Fd, const double* Q, const void* Yi,
+                          const double* const* metrics, const double* J, const void* D, void* Fh,
+                          const int64_t* tag, hipStream_t s);
+/* div, div.jl:16-54: U += J dt (sum of face-flux differences). */
+int f3d_div(f3d_ctx* ctx, double* U, const void* Fx, const void* Fy, const void* Fz,
+            const void* Fvx, const void* Fvy, const void* Fvz, double dt, const double* J,
+            const int64_t* tag, hipStream_t s);
+/* divSpecs, div.jl:61-84. */
+int f3d_div_specs(f3d_ctx* ctx, double* rhoi, const void* Fx, const void* Fy, const void* Fz,
+                  const void* Fdx, const void* Fdy, const void* Fdz, double dt, const double* J,
+                  const int64_t* tag, hipStream_t s);
+/* mixture, thermo.jl:371-403: Yi, lambda, mu, D at every padded point. */
+int f3d_mixture(f3d_ctx* ctx, const double* Q, const double* rhoi, void* Yi, void* lambda,
+                void* mu, void* D, const int64_t* tag, hipStream_t s);
+/* c2Prim, utils.jl:2-48 (FP64, e->T Newton). */
+int f3d_c2prim(f3d_ctx* ctx, double* U, double* Q, double* rhoi, const int64_t* tag,
+               hipStream_t s);
+/* prim2c, utils.jl:71-92. */
+int f3d_prim2c(f3d_ctx* ctx, double* U, const double* Q, hipStream_t s);
+/* linComb, utils.jl:95-112: U = a U + b Un on NV variables. */
+int f3d_lincomb(f3d_ctx* ctx, double* U, const double* Un, int32_t NV, double a, double b,
+                const int64_t* tag, hipStream_t s);
+/* getY, utils.jl:51-68 (Yi in double). */
+int f3d_get_y(f3d_ctx* ctx, double* Yi, const double* rhoi, const double* Q, const int64_t* tag,
+              hipStream_t s);
+/* ---- boundary fills (replace fillGhost/fillSpec, boundary.jl:180-191, and
+ * the MPI halo exchange mpi.jl:2-37 on a single GPU) ---------------------- */
+/* kinds[6] = {x-, x+, y-, y+, z-, z+} from F3D_BC_*; inflow_Q[7] and
+ * inflow_rhoi[Ns] (host) give the F3D_BC_INFLOW state. */
+int f3d_set_boundary(f3d_ctx* ctx, const int32_t* kinds, const double* inflow_Q,
+                     const double* inflow_rhoi);
+int f3d_fill(f3d_ctx* ctx, double* Q, double* U, double* rhoi, hipStream_t s);
+
+/* ---- chemistry (FP64) --------------------------------------------------- */
+/* eval_gpu_stiff, reactions.jl:235-309: one point-implicit step of dt. */
+int f3d_chem_stiff(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
+                   const int64_t* tag, hipStream_t s);
+/* eval_gpu, reactions.jl:176-232: one explicit Euler step of dt. */
+int f3d_chem_explicit(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
+                      const int64_t* tag, hipStream_t s);
+/* Backward-Euler Newton on (rho_1..rho_Ns, T) with a dense (Ns+1)x(Ns+1)
+ * pivoted LU per cell; iters [Nx*Ny*Nz] (device, may be NULL) receives the
+ * Newton iteration count of each interior cell.  No reference counterpart
+ * (extends eval_gpu_stiff, whose first iterate it reproduces). */
+int f3d_chem_newton(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
+                    int32_t maxiter, double rtol, double ttol, int32_t* iters,
+                    const int64_t* tag, hipStream_t s);
+
+/* ---- stage level -------------------------------------------------------- */
+/* Registers the block's metrics (9 device pointers, order as above) and J;
+ * the context keeps the pointers (and, for F3D_FP32, float copies). */
+int f3d_set_metrics(f3d_ctx* ctx, const double* const* metrics, const double* J);
+/* One Shu-Osher RK3 stage KRK = 1, 2, 3 exactly as solver.jl:298-326:
+ * (KRK == 1: Un <- U, rhon <- rhoi), mixture, shockSensor, specAdvance,
+ * flowAdvance, linComb (KRK 2, 3), c2Prim, fills.  Intermediates live in
+ * context-owned device memory (allocated on first use). */
+int f3d_rk3_stage(f3d_ctx* ctx, int32_t KRK, double* U, double* Un, double* rhoi, double* rhon,
+                  double* Q, double dt, const int64_t* tag, hipStream_t s);
+/* Device pointer of a context-owned intermediate after f3d_rk3_stage, by
+ * name ("Yi", "lambda", "mu", "D", "phi", "Fx", "Fy", "Fz", "Fvx", ...,
+ * "Fhz"); NULL if unknown. */
+void* f3d_workspace(f3d_ctx* ctx, const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLAME3D_H */
