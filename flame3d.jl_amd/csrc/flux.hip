@@ -103,6 +103,36 @@ __global__ void __launch_bounds__(kBlock)
   phi[p] = px + py + pz;
 }
 
+// WENO5-Z weights alpha_k = d_k (1 + (tau / (eps + beta_k))^2), d = (1, 6, 3)
+// (schemes.jl:152-155).  FP64 evaluates the reference expression as written.
+// In FP32, eps = 1e-26 makes (tau/(eps+beta_k))^2 overflow float whenever a
+// beta_k is ~0 (three equal stencil values) and tau > ~2e-7; the weights are
+// then computed as alpha_k * u with u = 1/(1 + T^2), T = tau/min_k q_k,
+// q_k = eps + beta_k:  alpha_k u = d_k (u + (1 - u) s_k^2), s_k = min q / q_k.
+// Every term is in [0, d_k] and the ratios alpha_k / sum alpha are unchanged.
+template <typename R>
+__device__ inline void zweights(R s11, R s22, R s33, R e, R& w1, R& w2, R& w3) {
+  const R tau = fabs(s11 - s33);
+  if constexpr (sizeof(R) == 8) {
+    R t = tau / (e + s11);
+    w1 = (R(1) + t * t) * R(1);
+    t = tau / (e + s22);
+    w2 = (R(1) + t * t) * R(6);
+    t = tau / (e + s33);
+    w3 = (R(1) + t * t) * R(3);
+  } else {
+    const R q1 = e + s11, q2 = e + s22, q3 = e + s33;
+    const R qm = fmin(q1, fmin(q2, q3));
+    const R T = tau / qm;
+    const R u = R(1) / (R(1) + T * T);
+    const R v = R(1) - u;
+    const R a = qm / q1, b = qm / q2, c = qm / q3;
+    w1 = (u + v * (a * a)) * R(1);
+    w2 = (u + v * (b * b)) * R(6);
+    w3 = (u + v * (c * c)) * R(3);
+  }
+}
+
 template <typename R>
 __device__ inline R minmod(R a, R b) {
   // schemes.jl:26-28
@@ -133,13 +163,7 @@ __device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm
     t = V3 - R(2) * V4 + V5;
     t2 = R(3) * V3 - R(4) * V4 + V5;
     R s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
-    R tau = fabs(s11 - s33);
-    t = tau / (e + s11);
-    s11 = (R(1) + t * t) * R(1);
-    t = tau / (e + s22);
-    s22 = (R(1) + t * t) * R(6);
-    t = tau / (e + s33);
-    s33 = (R(1) + t * t) * R(3);
+    zweights<R>(s11, s22, s33, e, s11, s22, s33);
     R invsum = R(1) / (s11 + s22 + s33);
     R v1 = R(2) * V1 - R(7) * V2 + R(11) * V3;
     R v2 = -V2 + R(5) * V3 + R(2) * V4;
@@ -160,13 +184,7 @@ __device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm
     t = V3 - R(2) * V2 + V1;
     t2 = R(3) * V3 - R(4) * V2 + V1;
     s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
-    tau = fabs(s11 - s33);
-    t = tau / (e + s11);
-    s11 = (R(1) + t * t) * R(1);
-    t = tau / (e + s22);
-    s22 = (R(1) + t * t) * R(6);
-    t = tau / (e + s33);
-    s33 = (R(1) + t * t) * R(3);
+    zweights<R>(s11, s22, s33, e, s11, s22, s33);
     invsum = R(1) / (s11 + s22 + s33);
     v1 = R(11) * V3 - R(7) * V4 + R(2) * V5;
     v2 = -V4 + R(5) * V3 + R(2) * V2;

@@ -259,3 +259,27 @@ def test_rk3_step(S, prec):
         print(f"  rk3 step {name} [{prec}] max rel err {max(errs):.2e}", flush=True)
         assert max(errs) <= tol, (name, errs)
     ctx.set_boundary((0,) * 6)
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_weno_fp32_flat_stencil_next_to_jump(S, d):
+    """Regression: four equal stencil values next to a jump make one WENO5-Z
+    smoothness indicator exactly 0, and (tau / (1e-26 + 0))^2 overflows float.
+    The FP32 weights must stay finite and match the FP64 oracle."""
+    ctx, orc = S["c32"], S["orc"]
+    Fp, Fm = orc.zeros(5), orc.zeros(5)
+    ax = (slice(None),) * d
+    for n in range(5):
+        Fp[..., n] = 0.0646 + 0.01 * n
+        Fm[..., n] = 0.0411 + 0.01 * n
+    jump = [slice(None)] * 3
+    jump[d] = slice(9, None)
+    Fp[tuple(jump) + (slice(None),)] += 0.02
+    Fm[tuple(jump) + (slice(None),)] -= 0.015
+    phi = orc.zeros() + 0.0046  # WENO branch everywhere
+    ref = orc.weno(d, phi, Fp, Fm, 5)
+    F = ctx.face_zeros(d, 5)
+    ctx.weno(d, F, _r(ctx, phi), _r(ctx, Fp), _r(ctx, Fm), 5)
+    got = to_np(F)
+    assert np.isfinite(got).all()
+    _check(got, ref, "fp32", f"WENO flat-next-to-jump d={d}")
