@@ -360,3 +360,25 @@ def chem_cells(N=(128, 128, 128), mech=None, seed=12345, dev="cpu",
     zero = torch.zeros(sh, dtype=torch.float64)
     Q, rhoi = state_from_TPY(mech, T, P, Y, zero, zero, zero)
     return Q.to(dev).contiguous(), rhoi.to(dev).contiguous()
+
+
+def chem_cells_air(N=(16, 8, 8), mech=None, seed=4, T_range=(3000.0, 6000.0),
+                   P_atm=(0.03, 1.0)):
+    """Dissociating 5-species air reactors: T ~ U[3000, 6000] K, P ~ U[0.03,
+    1] atm, composition air with a random partial dissociation."""
+    Nx, Ny, Nz = N
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    sh = (Nz + 2 * NG, Ny + 2 * NG, Nx + 2 * NG)
+    T = T_range[0] + (T_range[1] - T_range[0]) * torch.rand(sh, generator=g, dtype=torch.float64)
+    P = 101325.0 * (P_atm[0] + (P_atm[1] - P_atm[0]) * torch.rand(sh, generator=g,
+                                                                   dtype=torch.float64))
+    a = 0.3 * torch.rand(sh, generator=g, dtype=torch.float64)  # O2 dissociation fraction
+    b = 0.05 * torch.rand(sh, generator=g, dtype=torch.float64)  # N2 -> NO / N
+    Y = torch.zeros((mech.Ns,) + sh, dtype=torch.float64)
+    Y[mech.index("O2")] = 0.233 * (1 - a)
+    Y[mech.index("O")] = 0.233 * a
+    Y[mech.index("N2")] = 0.767 * (1 - b)
+    Y[mech.index("NO")] = 0.767 * b * 0.5
+    Y[mech.index("N")] = 0.767 * b * 0.5
+    zero = torch.zeros(sh, dtype=torch.float64)
+    return state_from_TPY(mech, T, P, Y, zero, zero, zero)

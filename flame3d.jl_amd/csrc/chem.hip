@@ -41,8 +41,10 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 }
 
 // vproductionRate(_Jac), reactions.jl:311-460.  A[l][m] accumulates
-// Arate[l, m] = d wdot_l / d c_m (third-body and T dependence neglected).
-template <int NS, bool JAC>
+// Arate[l, m] = d wdot_l / d c_m; the reference neglects the third-body and
+// T dependence, FULL (Newton mode) adds the [M] derivative of three-body and
+// falloff rates (Lindemann part).
+template <int NS, bool JAC, bool FULL = false>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
                                        double (&A)[NS][NS], double* wdot) {
   const double lgT = log(T);
@@ -74,12 +76,16 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       if (vr != 0) q_r *= ipow(sc[m], vr);
     }
     const int rt = P->rtype[n];
+    double dlnq_dM = 0.0;
+    double fac = 1.0;  // third-body / falloff factor multiplying both directions
     if (rt == 2) {
       double mix = 0.0;
 #pragma unroll
       for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
       q_f *= mix;
       q_r *= mix;
+      dlnq_dM = 1.0 / mix;
+      fac = mix;
     } else if (rt == 3) {
       double mix = 0.0;
 #pragma unroll
@@ -96,13 +102,57 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       const double k_1 = (redP / (1.0 + redP)) * F_troe;
       q_f *= k_1;
       q_r *= k_1;
+      dlnq_dM = 1.0 / ((1.0 + redP) * mix);
+      fac = k_1;
+    }
+    if (JAC && FULL && rt != 1) {
+      const double dqM = (q_f - q_r) * dlnq_dM;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const double e = P->ef[n][m];
+        if (e == 0.0) continue;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int dl = P->vr[n][l] - P->vf[n][l];
+          if (dl != 0) A[l][m] += dqM * e * (double)dl;
+        }
+      }
+    }
+    if (JAC && FULL) {
+      // exact mass-action derivatives: dq/dc_m = k nu_m c_m^(nu_m-1) prod_{j!=m} c_j^nu_j
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const int vfm = P->vf[n][m], vrm = P->vr[n][m];
+        if (vfm == 0 && vrm == 0) continue;
+        double dqf = 0.0, dqr = 0.0;
+        if (vfm != 0) {
+          double pr = kf * (double)vfm * ipow(sc[m], vfm - 1);
+#pragma unroll
+          for (int j = 0; j < NS; ++j)
+            if (j != m && P->vf[n][j] != 0) pr *= ipow(sc[j], P->vf[n][j]);
+          dqf = pr;
+        }
+        if (vrm != 0) {
+          double pr = kf / Kc * (double)vrm * ipow(sc[m], vrm - 1);
+#pragma unroll
+          for (int j = 0; j < NS; ++j)
+            if (j != m && P->vr[n][j] != 0) pr *= ipow(sc[j], P->vr[n][j]);
+          dqr = pr;
+        }
+        const double dd = (dqf - dqr) * fac;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int dl = P->vr[n][l] - P->vf[n][l];
+          if (dl != 0) A[l][m] += dd * (double)dl;
+        }
+      }
     }
     const double dq = q_f - q_r;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
       const int vf = P->vf[n][m], vr = P->vr[n][m];
       if (vf != vr) wdot[m] += dq * (double)(vr - vf);
-      if (JAC && (vf != 0 || vr != 0)) {
+      if (JAC && !FULL && (vf != 0 || vr != 0)) {
         const double invsc = 1 / (sc[m] + kEps);
         const double Awf = (double)vf * q_f * invsc;
         const double Awr = (double)vr * q_r * invsc;
@@ -262,7 +312,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int n = 0; n < NS; ++n) sc[n] = r[n] / P->mw[n] * 1e-6;
       double Ar[NS][NS], wdot2[NS];
-      production_rate<NS, true>(P, sc, T, Ar, wdot);
+      production_rate<NS, true, true>(P, sc, T, Ar, wdot);
       const double hT = 1e-6 * T;
       production_rate<NS, false>(P, sc, T + hT, Ar, wdot2);  // Ar untouched (JAC=false)
       double A[N1][N1], rhs[N1], dx[N1];

@@ -295,11 +295,11 @@ void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, doub
 // NULL (explicit path).  jac_eps is eps(Float64) in the reference
 // (reactions.jl:451) and 1e-20 in NN/H2/chemical_test.cpp:233.
 void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
-                        const double* sc, double T, double jac_eps, double* wdot,
-                        double* Arate) {
+                        const double* sc, double T, double jac_eps, int32_t full_jac,
+                        double* wdot, double* Arate) {
   Grid g(d);
   const int Ns = g.Ns, Nr = g.Nr;
-  std::vector<double> gi(Ns), kf(Nr), Kc(Nr);
+  std::vector<double> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
   for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
   if (Arate)
     for (int m = 0; m < Ns * Ns; ++m) Arate[m] = 0.0;
@@ -324,10 +324,14 @@ void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics
       if (vr != 0) q_r *= ipow(sc[m], vr);
     }
     double mixture = 0.0;
+    double dlnq_dM = 0.0;  // d ln(q) / d[M], used by the full Jacobian only
+    kfac_den[n] = 0.0;
     if (kin->reaction_type[n] == 2) {
       for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
       q_f *= mixture;
       q_r *= mixture;
+      dlnq_dM = 1.0 / mixture;
+      kfac_den[n] = mixture;
     } else if (kin->reaction_type[n] == 3) {
       for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
       const double* lp = kin->loP + 3 * n;
@@ -344,10 +348,52 @@ void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics
       double k_1 = (redP / (1.0 + redP)) * F_troe;
       q_f *= k_1;
       q_r *= k_1;
+      dlnq_dM = 1.0 / ((1.0 + redP) * mixture);
+      kfac_den[n] = k_1;
+    }
+    if (Arate && full_jac && dlnq_dM != 0.0) {
+      // third-body / falloff dependence through [M] = sum ef_m c_m
+      // (Lindemann part of the falloff; F_troe's dependence on Pr is neglected)
+      const double dq = (q_f - q_r) * dlnq_dM;
+      for (int m = 0; m < Ns; ++m) {
+        const double e = kin->ef[m + Ns * n];
+        if (e == 0.0) continue;
+        for (int l = 0; l < Ns; ++l)
+          Arate[l + Ns * m] += dq * e * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
+      }
+    }
+    if (Arate && full_jac) {
+      // exact concentration derivatives of the mass-action products:
+      // dq/dc_m = q0 * nu_m c_m^(nu_m-1) prod_{j != m} c_j^nu_j, q0 = q / prod
+      const double kfac = q_f, krac = q_r;  // after the [M] / falloff factors
+      for (int m = 0; m < Ns; ++m) {
+        long vfm = kin->vf[m + Ns * n], vrm = kin->vr[m + Ns * n];
+        if (vfm == 0 && vrm == 0) continue;
+        double dqf = 0.0, dqr = 0.0;
+        if (vfm != 0) {
+          double pr = kf[n] * (double)vfm * ipow(sc[m], vfm - 1);
+          for (int j = 0; j < Ns; ++j)
+            if (j != m && kin->vf[j + Ns * n] != 0) pr *= ipow(sc[j], kin->vf[j + Ns * n]);
+          dqf = pr;
+        }
+        if (vrm != 0) {
+          double pr = kf[n] / Kc[n] * (double)vrm * ipow(sc[m], vrm - 1);
+          for (int j = 0; j < Ns; ++j)
+            if (j != m && kin->vr[j + Ns * n] != 0) pr *= ipow(sc[j], kin->vr[j + Ns * n]);
+          dqr = pr;
+        }
+        // the third-body / falloff factor multiplies both directions
+        const double fac = (kfac_den[n] != 0.0) ? kfac_den[n] : 1.0;
+        const double dd = (dqf - dqr) * fac;
+        for (int l = 0; l < Ns; ++l)
+          Arate[l + Ns * m] += dd * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
+        (void)kfac;
+        (void)krac;
+      }
     }
     for (int m = 0; m < Ns; ++m) {
       wdot[m] += (q_f - q_r) * (double)(kin->vr[m + Ns * n] - kin->vf[m + Ns * n]);
-      if (Arate) {
+      if (Arate && !full_jac) {
         double invsc = 1 / (sc[m] + jac_eps);
         double Awf = (double)kin->vf[m + Ns * n] * q_f * invsc;
         double Awr = (double)kin->vr[m + Ns * n] * q_r * invsc;
@@ -1025,7 +1071,7 @@ void or_chem_stiff(const or_dims* d, double* U, double* Q, double* rhoi, double 
           r[n] = rhoi[p + n * NP];
           s[n] = r[n] / th->mw[n] * 1e-6;
         }
-        or_production_rate(d, th, kin, s, T, kEps, wdot, Arate.data());
+        or_production_rate(d, th, kin, s, T, kEps, 0, wdot, Arate.data());
         for (int l = 0; l < Ns; ++l)
           for (int n = 0; n < Ns; ++n)
             A1[n + Ns * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * dt;
@@ -1063,7 +1109,7 @@ void or_chem_explicit(const or_dims* d, double* U, double* Q, double* rhoi, doub
           r[n] = rhoi[p + n * NP];
           s[n] = r[n] / th->mw[n] * 1e-6;
         }
-        or_production_rate(d, th, kin, s, T, kEps, wdot, nullptr);
+        or_production_rate(d, th, kin, s, T, kEps, 0, wdot, nullptr);
         double dei = 0;
         for (int n = 0; n < Ns; ++n) {
           double drho = wdot[n] * th->mw[n] * 1e6 * dt;
@@ -1118,9 +1164,9 @@ void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double
         int it = 0;
         for (; it < maxiter;) {
           for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
-          or_production_rate(d, th, kin, s, T, kEps, wdot, Arate.data());
+          or_production_rate(d, th, kin, s, T, kEps, 1, wdot, Arate.data());
           double hT = 1e-6 * T;
-          or_production_rate(d, th, kin, s, T + hT, kEps, wdot2, nullptr);
+          or_production_rate(d, th, kin, s, T + hT, kEps, 1, wdot2, nullptr);
           // residual (negated) and Jacobian
           for (int n = 0; n < Ns; ++n) {
             double src = wdot[n] * th->mw[n] * 1e6 * dt;
@@ -1225,7 +1271,7 @@ void or_reactor0d_step(const or_dims* d, const or_thermo* th, const or_kinetics*
   double s[64], wdot[64], b[64], x[64];
   std::vector<double> Ar(Ns * Ns), A1(Ns * Ns);
   for (int n = 0; n < Ns; ++n) s[n] = rhoi[n] / th->mw[n] * 1e-6;
-  or_production_rate(d, th, kin, s, T, jac_eps, wdot, Ar.data());
+  or_production_rate(d, th, kin, s, T, jac_eps, 0, wdot, Ar.data());
   for (int i = 0; i < Ns; ++i)
     for (int j = 0; j < Ns; ++j) {
       A1[i + Ns * j] = (i == j ? 1.0 : 0.0);

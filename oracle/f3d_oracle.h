@@ -50,7 +50,7 @@ double or_get_T(const or_dims* d, const or_thermo* th, double e, const double* r
 void or_h_specs(const or_dims* d, const or_thermo* th, double T, double* h);
 void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g);
 void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
-                        const double* sc, double T, double jac_eps,
+                        const double* sc, double T, double jac_eps, int32_t full_jac,
                         double* wdot, double* Arate /* [Ns,Ns] or NULL */);
 void or_gauss_solve(int32_t n, double* a /* [n,n] col-major, destroyed */,
                     double* b /* destroyed */, double* x);

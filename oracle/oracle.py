@@ -148,12 +148,13 @@ class Oracle:
         self.lib.or_gibbs(C.byref(self.dims), C.byref(self.thermo), C.c_double(T), _f(g))
         return g
 
-    def production_rate(self, sc, T, jac_eps=2.220446049250313e-16, jac=True):
+    def production_rate(self, sc, T, jac_eps=2.220446049250313e-16, jac=True, full_jac=False):
         sc = np.ascontiguousarray(sc, dtype=np.float64)
         w = np.zeros(self.Ns)
         A = np.zeros((self.Ns, self.Ns), order="F") if jac else None
         self.lib.or_production_rate(C.byref(self.dims), C.byref(self.thermo), C.byref(self.kin),
-                                    _f(sc), C.c_double(T), C.c_double(jac_eps), _f(w),
+                                    _f(sc), C.c_double(T), C.c_double(jac_eps),
+                                    C.c_int32(1 if full_jac else 0), _f(w),
                                     _f(A) if jac else None)
         return (w, A) if jac else w
 
