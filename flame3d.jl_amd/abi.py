@@ -53,6 +53,21 @@ class Scheme(C.Structure):
                 ("precision", C.c_int32), ("bug_compat", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
+NN_NPARAM = 12512  # F3D_NN_NPARAM
+
+
+class NNConfig(C.Structure):
+    _fields_ = [("in_mean", C.c_float * 11), ("in_std", C.c_float * 11),
+                ("out_mean", C.c_float * 9), ("out_std", C.c_float * 9),
+                ("mahal_mean", C.c_float * 11), ("mahal_inv", C.c_float * 121),
+                ("d_max", C.c_float), ("elem_tol", C.c_float), ("neg_tol", C.c_float),
+                ("res_tol", C.c_float),
+                ("boxcox_lambda", C.c_float), ("label_mode", C.c_int32),
+                ("label_scale", C.c_float * 9),
+                ("n_elem", C.c_int32),
+                ("elem", (C.c_float * 9) * 4), ("dt", C.c_double)]
+
+
 # exported symbols and their argument types (all return int unless noted)
 _PROTOS = {
     "f3d_create": [C.POINTER(Dims), C.POINTER(Thermo), C.POINTER(Kinetics), C.POINTER(Scheme),
@@ -78,6 +93,14 @@ _PROTOS = {
     "f3d_chem_explicit": [_vp, _vp, _vp, _vp, C.c_double, _vp, _vp],
     "f3d_chem_newton": [_vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_double, C.c_double,
                         _vp, _vp, _vp],
+    "f3d_nn_init": [_vp, C.POINTER(NNConfig)],
+    "f3d_nn_set_params": [_vp, _vp],
+    "f3d_nn_get_params": [_vp, _vp],
+    "f3d_nn_train_step": [_vp, _vp, _vp, C.c_int32, C.c_float, _vp, _vp],
+    "f3d_nn_grad": [_vp, _vp, _vp, C.c_int32, _vp, _vp],
+    "f3d_nn_forward": [_vp, _vp, _vp, C.c_int32, _vp],
+    "f3d_nn_chem": [_vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_double, C.c_double, _vp,
+                    C.POINTER(C.c_int64), _vp, _vp],
     "f3d_set_metrics": [_vp, _vp, _vp],
     "f3d_rk3_stage": [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp],
     "f3d_workspace": [_vp, C.c_char_p],
@@ -338,6 +361,67 @@ class Context:
         _rc(self.lib.f3d_chem_newton(self.h, self._pad(U, 5, "U"), self._pad(Q, 7, "Q"),
                                      self._pad(rhoi, self.Ns, "rhoi"), dt, maxiter, rtol, ttol,
                                      it, self._tag(tag), self.stream()), "f3d_chem_newton")
+
+    # ---- learned chemistry surrogate ------------------------------------
+    def nn_init(self, cfg: "NNConfig"):
+        _rc(self.lib.f3d_nn_init(self.h, C.byref(cfg)), "f3d_nn_init")
+        self._nn_cfg = cfg
+
+    def nn_set_params(self, params):
+        a = np.ascontiguousarray(np.asarray(params, dtype=np.float32).ravel())
+        if a.size != NN_NPARAM:
+            raise F3DError(f"params: expected {NN_NPARAM} floats, got {a.size}")
+        _rc(self.lib.f3d_nn_set_params(self.h, a.ctypes.data), "f3d_nn_set_params")
+
+    def nn_get_params(self):
+        a = np.zeros(NN_NPARAM, np.float32)
+        _rc(self.lib.f3d_nn_get_params(self.h, a.ctypes.data), "f3d_nn_get_params")
+        return a
+
+    def _nn_batch(self, X, Y, ny):
+        t = self.torch
+        if X.dim() != 2 or X.shape[1] != 11:
+            raise F3DError(f"X: expected (B, 11), got {tuple(X.shape)}")
+        B = X.shape[0]
+        self._chk(X, (B, 11), t.float32, "X")
+        self._chk(Y, (B, ny), t.float32, "Y")
+        return B
+
+    def nn_train_step(self, X, Y, lr, loss=None):
+        """One Adam step; loss: optional device float32 tensor of one element."""
+        B = self._nn_batch(X, Y, 9)
+        lp = None
+        if loss is not None:
+            lp = self._chk(loss, (1,), self.torch.float32, "loss")
+        _rc(self.lib.f3d_nn_train_step(self.h, X.data_ptr(), Y.data_ptr(), B, lr, lp,
+                                       self.stream()), "f3d_nn_train_step")
+
+    def nn_grad(self, X, Y):
+        """Gradient (NN_NPARAM floats) and loss of the batch, as a device tensor."""
+        B = self._nn_batch(X, Y, 9)
+        g = self.torch.empty(NN_NPARAM + 1, dtype=self.torch.float32, device=X.device)
+        _rc(self.lib.f3d_nn_grad(self.h, X.data_ptr(), Y.data_ptr(), B, g.data_ptr(),
+                                 self.stream()), "f3d_nn_grad")
+        return g
+
+    def nn_forward(self, X):
+        Y = self.torch.empty((X.shape[0], 9), dtype=self.torch.float32, device=X.device)
+        B = self._nn_batch(X, Y, 9)
+        _rc(self.lib.f3d_nn_forward(self.h, X.data_ptr(), Y.data_ptr(), B, self.stream()),
+            "f3d_nn_forward")
+        return Y
+
+    def nn_chem(self, U, Q, rhoi, dt, maxiter=50, rtol=1e-10, ttol=1e-6, status=None,
+                tag=None):
+        """Surrogate chemistry step with Newton fallback; returns the fallback count."""
+        st = None
+        if status is not None:
+            st = self._chk(status, (self.Nz, self.Ny, self.Nx), self.torch.int8, "status")
+        nfb = C.c_int64(0)
+        _rc(self.lib.f3d_nn_chem(self.h, self._pad(U, 5, "U"), self._pad(Q, 7, "Q"),
+                                 self._pad(rhoi, self.Ns, "rhoi"), dt, maxiter, rtol, ttol, st,
+                                 C.byref(nfb), self._tag(tag), self.stream()), "f3d_nn_chem")
+        return int(nfb.value)
 
     # ---- stage level -----------------------------------------------------
     def set_metrics(self, metrics, J):

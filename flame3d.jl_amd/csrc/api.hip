@@ -242,6 +242,7 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
 
 int f3d_destroy(f3d_ctx* c) {
   if (!c) return F3D_OK;
+  nn_free(c);
   if (c->ws_base) (void)hipFree(c->ws_base);
   if (c->dparams) (void)hipFree(c->dparams);
   delete c;

@@ -1,0 +1,214 @@
+// FP64 reaction kinetics shared by the chemistry kernels: integer powers,
+// Gibbs energies, production rates with the analytic species Jacobian
+// (reactions.jl:311-460) and the pivoted dense solve (reactions.jl:463-503).
+#pragma once
+
+#include "f3d_internal.h"
+
+namespace f3d {
+
+__device__ inline double ipow(double x, int n) {
+  // Julia Float64^Int for the stoichiometric/order exponents that occur (|n| <= 3)
+  if (n == 0) return 1.0;
+  if (n < 0) {
+    double r = x;
+    for (int m = 1; m < -n; ++m) r = r * x;
+    return 1.0 / r;
+  }
+  double r = x;
+  for (int m = 1; m < n; ++m) r = r * x;
+  return r;
+}
+
+// gibbs, thermo.jl:163-191 (g/RT)
+template <int NS>
+__device__ inline void gibbs(const DevParams* __restrict__ P, double T, double lgT, double invT,
+                             double* gi) {
+  const double mlogT = 1 - lgT;
+  const double T1 = 0.5 * T;
+  const double T2 = 1.0 / 6.0 * T * T;
+  const double T3 = T1 * T2;
+  const double T4 = 1.8 * T2 * T2;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const double* c = (T < P->sep[n]) ? P->lo[n] : P->hi[n];
+    gi[n] = c[0] * mlogT - c[1] * T1 - c[2] * T2 - c[3] * T3 - c[4] * T4 + c[5] * invT - c[6];
+  }
+}
+
+// vproductionRate(_Jac), reactions.jl:311-460.  A[l][m] accumulates
+// Arate[l, m] = d wdot_l / d c_m; the reference neglects the third-body and
+// T dependence, FULL (Newton mode) adds the [M] derivative of three-body and
+// falloff rates (Lindemann part).
+template <int NS, bool JAC, bool FULL = false>
+__device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
+                                       double (&A)[NS][NS], double* wdot) {
+  const double lgT = log(T);
+  const double invT = 1.0 / T;
+  double gi[NS];
+  gibbs<NS>(P, T, lgT, invT, gi);
+  const double RsT = P->Ru / P->atm * 1e6 * T;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    wdot[m] = 0.0;
+    if (JAC) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) A[l][m] = 0.0;
+    }
+  }
+  const int Nr = P->Nr;
+  for (int n = 0; n < Nr; ++n) {
+    const double kf = P->Arr[n][0] * exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+    double dg = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
+    const double Kc = ipow(RsT, P->sgm[n]) * exp(dg);
+    double q_f = kf;
+    double q_r = q_f / Kc;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      if (vf != 0) q_f *= ipow(sc[m], vf);
+      if (vr != 0) q_r *= ipow(sc[m], vr);
+    }
+    const int rt = P->rtype[n];
+    double dlnq_dM = 0.0;
+    double fac = 1.0;  // third-body / falloff factor multiplying both directions
+    if (rt == 2) {
+      double mix = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      q_f *= mix;
+      q_r *= mix;
+      dlnq_dM = 1.0 / mix;
+      fac = mix;
+    } else if (rt == 3) {
+      double mix = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      const double redP = mix / kf * P->loP[n][0] * exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      const double logPred = log10(redP);
+      const double A_ = P->Troe[n][0];
+      const double logFcent = log10((1 - A_) * exp(-T / P->Troe[n][1]) +
+                                    A_ * exp(-T / P->Troe[n][2]) + exp(-P->Troe[n][3] * invT));
+      const double troe_c = -0.4 - 0.67 * logFcent;
+      const double troe_n = 0.75 - 1.27 * logFcent;
+      const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
+      const double F_troe = pow(10.0, logFcent / (1.0 + troe * troe));
+      const double k_1 = (redP / (1.0 + redP)) * F_troe;
+      q_f *= k_1;
+      q_r *= k_1;
+      dlnq_dM = 1.0 / ((1.0 + redP) * mix);
+      fac = k_1;
+    }
+    if (JAC && FULL && rt != 1) {
+      const double dqM = (q_f - q_r) * dlnq_dM;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const double e = P->ef[n][m];
+        if (e == 0.0) continue;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int dl = P->vr[n][l] - P->vf[n][l];
+          if (dl != 0) A[l][m] += dqM * e * (double)dl;
+        }
+      }
+    }
+    if (JAC && FULL) {
+      // exact mass-action derivatives: dq/dc_m = k nu_m c_m^(nu_m-1) prod_{j!=m} c_j^nu_j
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const int vfm = P->vf[n][m], vrm = P->vr[n][m];
+        if (vfm == 0 && vrm == 0) continue;
+        double dqf = 0.0, dqr = 0.0;
+        if (vfm != 0) {
+          double pr = kf * (double)vfm * ipow(sc[m], vfm - 1);
+#pragma unroll
+          for (int j = 0; j < NS; ++j)
+            if (j != m && P->vf[n][j] != 0) pr *= ipow(sc[j], P->vf[n][j]);
+          dqf = pr;
+        }
+        if (vrm != 0) {
+          double pr = kf / Kc * (double)vrm * ipow(sc[m], vrm - 1);
+#pragma unroll
+          for (int j = 0; j < NS; ++j)
+            if (j != m && P->vr[n][j] != 0) pr *= ipow(sc[j], P->vr[n][j]);
+          dqr = pr;
+        }
+        const double dd = (dqf - dqr) * fac;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int dl = P->vr[n][l] - P->vf[n][l];
+          if (dl != 0) A[l][m] += dd * (double)dl;
+        }
+      }
+    }
+    const double dq = q_f - q_r;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      if (vf != vr) wdot[m] += dq * (double)(vr - vf);
+      if (JAC && !FULL && (vf != 0 || vr != 0)) {
+        const double invsc = 1 / (sc[m] + kEps);
+        const double Awf = (double)vf * q_f * invsc;
+        const double Awr = (double)vr * q_r * invsc;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int dl = P->vr[n][l] - P->vf[n][l];
+          if (dl != 0) A[l][m] += (Awf - Awr) * (double)dl;
+        }
+      }
+    }
+  }
+}
+
+// solve, reactions.jl:463-503 (partial pivoting; row exchanges done with
+// selects so the matrix stays in registers).  Only the columns >= k that
+// the back substitution reads are eliminated; they are bit-identical to the
+// reference's full-row update.
+template <int N>
+__device__ inline void gauss_solve(double (&a)[N][N], double (&b)[N], double (&x)[N]) {
+#pragma unroll
+  for (int k = 0; k < N - 1; ++k) {
+    double fmax_ = fabs(a[k][k]);
+    int kk = k;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double absa = fabs(a[i][k]);
+      if (absa > fmax_) {
+        fmax_ = absa;
+        kk = i;
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const bool sw = (kk == i);
+#pragma unroll
+      for (int j = k; j < N; ++j) {
+        const double t = a[k][j];
+        a[k][j] = sw ? a[i][j] : t;
+        a[i][j] = sw ? t : a[i][j];
+      }
+      const double t = b[k];
+      b[k] = sw ? b[i] : t;
+      b[i] = sw ? t : b[i];
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double f = -a[i][k] / a[k][k];
+#pragma unroll
+      for (int j = k + 1; j < N; ++j) a[i][j] += f * a[k][j];
+      b[i] += f * b[k];
+    }
+  }
+  x[N - 1] = b[N - 1] / a[N - 1][N - 1];
+#pragma unroll
+  for (int i = N - 2; i >= 0; --i) {
+    double xi = b[i];
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) xi -= a[i][j] * x[j];
+    x[i] = xi / a[i][i];
+  }
+}
+
+}  // namespace f3d

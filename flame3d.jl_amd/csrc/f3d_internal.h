@@ -62,11 +62,26 @@ struct f3d_ctx {
   void* ws_base = nullptr;
   size_t ws_bytes = 0;
   void* ws[32] = {nullptr};
+  // chemistry surrogate (f3d_nn_*)
+  bool nn_ready = false;
+  float* nn_master = nullptr;  // FP32 parameters, then Adam moments m1, m2
+  float* nn_m1 = nullptr;
+  float* nn_m2 = nullptr;
+  void* nn_Wb = nullptr;   // BF16 weights [o][k]
+  void* nn_Wtb = nullptr;  // BF16 weights transposed [k][o]
+  float* nn_partial = nullptr;
+  long nn_partial_tiles = 0;
+  void* nn_cfg = nullptr;  // device NNDev
+  double nn_dt = 0;
+  long nn_step = 0;
+  int32_t* nn_fb = nullptr;  // fallback cell list [Nx*Ny*Nz] and its counter
+  int32_t* nn_fbcount = nullptr;
 };
 
 namespace f3d {
 
 void set_error(const std::string& msg);
+void nn_free(f3d_ctx* c);
 int hip_check(hipError_t e, const char* what);
 
 // ---- launch helpers ---------------------------------------------------------
@@ -233,6 +248,32 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
   return t1;
 }
 
+// common epilogue of eval_gpu / eval_gpu_stiff (reactions.jl:209-229, :286-306)
+template <int NS>
+__device__ inline void chem_epilogue(const DevParams* __restrict__ P, double* __restrict__ U,
+                                     double* __restrict__ Q, double* __restrict__ rhoi, long p,
+                                     long NP, double* r, double dei) {
+  U[p + 4 * NP] += dei;
+  const double rho = fmax(Q[p], kEps);
+  double sr = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    r[n] = fmax(r[n], 0.0);
+    sr += r[n];
+  }
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r[n] *= rho / sr;
+  double ein = Q[p + 6 * NP];
+  ein += dei;
+  const double T = fmax(get_T<NS>(P, ein, r, nullptr), kEps);
+  const double pr = fmax(pmixture<NS>(P, T, r), kEps);
+  Q[p + 4 * NP] = pr;
+  Q[p + 5 * NP] = T;
+  Q[p + 6 * NP] = ein;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = r[n];
+}
+
 // ---- internal launchers (defined in the .hip files) ------------------------
 int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
                       const double* Ay, const double* Az, const int64_t* tag, hipStream_t s);
@@ -266,6 +307,6 @@ int launch_get_y(f3d_ctx* c, double* Yi, const double* rhoi, const double* Q,
 int launch_fill(f3d_ctx* c, double* Q, double* U, double* rhoi, hipStream_t s);
 int launch_chem(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,
                 int maxiter, double rtol, double ttol, int32_t* iters, const int64_t* tag,
-                hipStream_t s);
+                hipStream_t s, const int32_t* list = nullptr, long nlist = 0);
 
 }  // namespace f3d

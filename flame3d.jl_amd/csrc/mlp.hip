@@ -1,0 +1,798 @@
+// Learned chemistry surrogate: an 11 -> 64 -> 64 -> 64 -> 9 MLP with GELU
+// (tanh form), BF16 weights and activations on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, trained on the GPU by the
+// kernels below (forward, backward, Adam) and evaluated in a fused chemistry
+// kernel that gates each cell (Mahalanobis distance of its inputs, element
+// balance of the prediction) and hands rejected cells to the FP64 Newton
+// integrator.  The reference pattern is reactions.jl:2-72 (pre_input,
+// evalModel, post_predict); the network shape and data follow the 128^3
+// H2/O2 chemistry workload.
+//
+// MFMA operand layout of v_mfma_f32_16x16x32_bf16 (verified on the device):
+// lane l holds A[l%16][8*(l/16) + i] and B[8*(l/16) + i][l%16] (i = 0..7) and
+// receives C[4*(l/16) + i][l%16] (i = 0..3).  Activations are staged in LDS in
+// both [sample][feature] and [feature][sample] order so that every operand is
+// a contiguous 16-byte read.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+
+#include "f3d_chem.h"
+#include "f3d_internal.h"
+#include "f3d_nn.h"
+
+namespace f3d {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+__device__ inline float gelu(float x, float& dg) {
+  const float c = 0.7978845608028654f, k = 0.044715f;
+  const float u = c * (x + k * x * x * x);
+  const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));  // tanh(u), saturates cleanly
+  dg = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * k * x * x);
+  return 0.5f * x * (1.0f + t);
+}
+
+// ---------------------------------------------------------------------------
+// Training tile: 64 samples per workgroup, 4 waves (wave w owns samples
+// 16w..16w+15 on the data path).  Writes this tile's weight/bias gradient of
+// the loss L = sum (y - t)^2 / (B * NOUT) and its partial loss to
+// partial[blockIdx.x][0 .. NPARAM].
+// ---------------------------------------------------------------------------
+struct TileLds {
+  __bf16 X[64][NN_INP];    // [s][k], forward operand of layer 0
+  __bf16 Xt[NN_INP][64];   // [k][s], weight-gradient operand of layer 0
+  __bf16 At[3][NN_H][64];  // hidden activations [o][s] (weight-gradient operands)
+  union {
+    __bf16 A[2][64][NN_H];  // forward ping-pong [s][o]
+    struct {
+      __bf16 dZ[64][NN_H];   // backward: dZ [s][o] (data-gradient operand)
+      __bf16 dZt[NN_H][64];  // dZ^T [o][s] (weight-gradient operand)
+    } g;
+  } u;
+  float bsum[4][NN_H];  // per-wave bias-gradient partial sums
+  float red[256];
+};
+
+// sum over the four 16-lane groups holding the same column (lanes r, r+16, r+32, r+48)
+__device__ inline float sum_q(float v) {
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_nn_train_tile(const float* __restrict__ Xg, const float* __restrict__ Tg, int B,
+                    const __bf16* __restrict__ Wb, const __bf16* __restrict__ Wtb,
+                    const float* __restrict__ master, float* __restrict__ partial) {
+  __shared__ __align__(16) TileLds L;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, q = lane >> 4;
+  const long s0 = (long)blockIdx.x * 64;
+  for (int e = tid; e < 64 * NN_INP; e += 256) {
+    const int s = e / NN_INP, k = e % NN_INP;
+    const float v = (k < NN_IN) ? Xg[(s0 + s) * NN_IN + k] : 0.0f;
+    L.X[s][k] = (__bf16)v;
+    L.Xt[k][s] = (__bf16)v;
+  }
+  __syncthreads();
+  const int sw = 16 * w;  // this wave's first sample on the data path
+  float gp[3][4][4];      // GELU' of the wave's hidden pre-activations
+  // ---- forward, hidden layers ----
+  for (int l = 0; l < 3; ++l) {
+    const int NI = nn_in_dim(l);
+    const __bf16* W = Wb + nn_w_off(l);
+    const float* bias = master + nn_b_off(l);
+#pragma unroll
+    for (int ot = 0; ot < NN_H / 16; ++ot) {
+      f32x4 acc = {0, 0, 0, 0};
+      for (int kk = 0; kk < NI / 32; ++kk) {
+        const bf16x8 a = (l == 0) ? ld8(&L.X[sw + r16][32 * kk + 8 * q])
+                                  : ld8(&L.u.A[(l - 1) & 1][sw + r16][32 * kk + 8 * q]);
+        const bf16x8 b = ld8(W + (16 * ot + r16) * NI + 32 * kk + 8 * q);
+        acc = mfma(a, b, acc);
+      }
+      const int o = 16 * ot + r16;
+      const float bo = bias[o];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int s = sw + 4 * q + i;
+        float d;
+        const __bf16 a = (__bf16)gelu(acc[i] + bo, d);
+        gp[l][ot][i] = d;
+        L.u.A[l & 1][s][o] = a;
+        L.At[l][o][s] = a;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- output layer and dL/dy (the output layer is linear: dZ4 = dL/dy) ----
+  float lsum = 0.0f;
+  f32x4 yv[NN_OUTP / 16];
+  {
+    const __bf16* W = Wb + nn_w_off(3);
+#pragma unroll
+    for (int ot = 0; ot < NN_OUTP / 16; ++ot) {
+      f32x4 acc = {0, 0, 0, 0};
+      for (int kk = 0; kk < NN_H / 32; ++kk) {
+        const bf16x8 a = ld8(&L.u.A[0][sw + r16][32 * kk + 8 * q]);
+        const bf16x8 b = ld8(W + (16 * ot + r16) * NN_H + 32 * kk + 8 * q);
+        acc = mfma(a, b, acc);
+      }
+      yv[ot] = acc;
+    }
+  }
+  __syncthreads();  // A[0] (layer-2 output) fully read before dZ overwrites the union
+  {
+    const float* bias = master + nn_b_off(3);
+    const float scale = 2.0f / ((float)B * (float)NN_OUT);
+#pragma unroll
+    for (int ot = 0; ot < NN_OUTP / 16; ++ot) {
+      const int o = 16 * ot + r16;
+      float bs = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int s = sw + 4 * q + i;
+        float g = 0.0f;
+        if (o < NN_OUT) {
+          const float d = yv[ot][i] + bias[o] - Tg[(s0 + s) * NN_OUT + o];
+          lsum += d * d;
+          g = scale * d;
+        }
+        bs += g;
+        L.u.g.dZ[s][o] = (__bf16)g;
+        L.u.g.dZt[o][s] = (__bf16)g;
+      }
+      bs = sum_q(bs);
+      if (q == 0) L.bsum[w][o] = bs;
+    }
+  }
+  __syncthreads();
+  float* part = partial + (long)blockIdx.x * (NN_NPARAM + 1);
+  // ---- backward, layers 3..0 ----
+  for (int l = 3; l >= 0; --l) {
+    const int NI = nn_in_dim(l), NO = nn_out_dim(l);
+    for (int o = tid; o < NO; o += 256)
+      part[nn_b_off(l) + o] = L.bsum[0][o] + L.bsum[1][o] + L.bsum[2][o] + L.bsum[3][o];
+    // weight gradient dW[o][k] = sum_s dZ[s][o] A_prev[s][k], output tiles over the waves
+    const int nto = NO / 16, ntk = NI / 16;
+    for (int t = w; t < nto * ntk; t += 4) {
+      const int ot = t / ntk, kt = t % ntk;
+      f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const bf16x8 a = ld8(&L.u.g.dZt[16 * ot + r16][32 * ks + 8 * q]);
+        const bf16x8 b = (l == 0) ? ld8(&L.Xt[16 * kt + r16][32 * ks + 8 * q])
+                                  : ld8(&L.At[l - 1][16 * kt + r16][32 * ks + 8 * q]);
+        acc = mfma(a, b, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        part[nn_w_off(l) + (16 * ot + 4 * q + i) * NI + 16 * kt + r16] = acc[i];
+    }
+    if (l == 0) break;
+    // data gradient dA_prev[s][k] = sum_o dZ[s][o] W[o][k]  (W^T copy: [k][o])
+    f32x4 dA[4];
+    const __bf16* Wt = Wtb + nn_w_off(l);
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+      f32x4 acc = {0, 0, 0, 0};
+      for (int ks = 0; ks < NO / 32; ++ks) {
+        const bf16x8 a = ld8(&L.u.g.dZ[sw + r16][32 * ks + 8 * q]);
+        const bf16x8 b = ld8(Wt + (16 * kt + r16) * NO + 32 * ks + 8 * q);
+        acc = mfma(a, b, acc);
+      }
+      dA[kt] = acc;
+    }
+    __syncthreads();  // all reads of dZ, dZ^T and bsum of layer l are done
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+      const int k = 16 * kt + r16;
+      float bs = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int s = sw + 4 * q + i;
+        const float g = dA[kt][i] * gp[l - 1][kt][i];
+        bs += g;
+        L.u.g.dZ[s][k] = (__bf16)g;
+        L.u.g.dZt[k][s] = (__bf16)g;
+      }
+      bs = sum_q(bs);
+      if (q == 0) L.bsum[w][k] = bs;
+    }
+    __syncthreads();
+  }
+  L.red[tid] = lsum;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) L.red[tid] += L.red[tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) part[NN_NPARAM] = L.red[0] / ((float)B * (float)NN_OUT);
+}
+
+// Sum of the per-tile partials (gradient and loss), one thread per slot.
+__global__ void __launch_bounds__(256)
+    k_nn_reduce(const float* __restrict__ partial, int ntiles, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > NN_NPARAM) return;
+  float s = 0.0f;
+  for (int t = 0; t < ntiles; ++t) s += partial[(long)t * (NN_NPARAM + 1) + p];
+  out[p] = s;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the BF16
+// copies (row-major and transposed).  Padded weights stay zero: their
+// gradient is zero and they start at zero.
+__global__ void __launch_bounds__(256)
+    k_nn_adam(const float* __restrict__ partial, int ntiles, float* __restrict__ master,
+              float* __restrict__ m1, float* __restrict__ m2, __bf16* __restrict__ Wb,
+              __bf16* __restrict__ Wtb, float lr, float b1, float b2, float eps, float bc1,
+              float bc2, float* __restrict__ loss_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > NN_NPARAM) return;
+  float g = 0.0f;
+  for (int t = 0; t < ntiles; ++t) g += partial[(long)t * (NN_NPARAM + 1) + p];
+  if (p == NN_NPARAM) {
+    if (loss_out) *loss_out = g;
+    return;
+  }
+  const float a = b1 * m1[p] + (1.0f - b1) * g;
+  const float v = b2 * m2[p] + (1.0f - b2) * g * g;
+  m1[p] = a;
+  m2[p] = v;
+  const float wv = master[p] - lr * (a / bc1) / (sqrtf(v / bc2) + eps);
+  master[p] = wv;
+  int l, o, k;
+  if (nn_locate_weight(p, l, o, k)) {
+    Wb[p] = (__bf16)wv;
+    Wtb[nn_w_off(l) + k * nn_out_dim(l) + o] = (__bf16)wv;
+  }
+}
+
+// refresh the BF16 copies from the FP32 master (after f3d_nn_set_params)
+__global__ void __launch_bounds__(256)
+    k_nn_cast(const float* __restrict__ master, __bf16* __restrict__ Wb, __bf16* __restrict__ Wtb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NN_NPARAM) return;
+  int l, o, k;
+  if (nn_locate_weight(p, l, o, k)) {
+    Wb[p] = (__bf16)master[p];
+    Wtb[nn_w_off(l) + k * nn_out_dim(l) + o] = (__bf16)master[p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward of 64 samples by one wave (inference).  One 8 KB LDS buffer per
+// wave holds, in turn, the BF16 input tile, each hidden activation tile and
+// the FP32 outputs: a layer's 4x4 (or 4x2) 16x16 output tiles stay in
+// registers until every lane has read the previous activations.
+// ---------------------------------------------------------------------------
+union FwdLds {
+  __bf16 X[64][NN_INP];
+  __bf16 A[64][NN_H];
+  float Y[64][NN_OUTP];
+};
+
+// one dense layer for the wave's 64 samples: NI inputs, NO outputs (compile-time)
+template <int NI, int NO, bool LAST>
+__device__ __forceinline__ void nn_layer_wave(FwdLds& L, const __bf16* __restrict__ W,
+                                     const float* __restrict__ bias, int lane) {
+  const int r16 = lane & 15, q = lane >> 4;
+  f32x4 acc[4][NO / 16];
+#pragma unroll
+  for (int ot = 0; ot < NO / 16; ++ot) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) acc[st][ot] = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < NI / 32; ++kk) {
+      const bf16x8 b = ld8(W + (16 * ot + r16) * NI + 32 * kk + 8 * q);
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const __bf16* src = (NI == NN_INP) ? &L.X[16 * st + r16][32 * kk + 8 * q]
+                                           : &L.A[16 * st + r16][32 * kk + 8 * q];
+        acc[st][ot] = mfma(ld8(src), b, acc[st][ot]);
+      }
+    }
+  }
+  __syncthreads();  // every lane has read this layer's input tile
+#pragma unroll
+  for (int ot = 0; ot < NO / 16; ++ot) {
+    const int o = 16 * ot + r16;
+    const float bo = bias[o];
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int s = 16 * st + 4 * q + i;
+        if (LAST) {
+          L.Y[s][o] = acc[st][ot][i] + bo;
+        } else {
+          float d;
+          L.A[s][o] = (__bf16)gelu(acc[st][ot][i] + bo, d);
+        }
+      }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void nn_forward_wave(FwdLds& L, const __bf16* __restrict__ Wb,
+                                       const float* __restrict__ master, int lane) {
+  nn_layer_wave<NN_INP, NN_H, false>(L, Wb + nn_w_off(0), master + nn_b_off(0), lane);
+  nn_layer_wave<NN_H, NN_H, false>(L, Wb + nn_w_off(1), master + nn_b_off(1), lane);
+  nn_layer_wave<NN_H, NN_H, false>(L, Wb + nn_w_off(2), master + nn_b_off(2), lane);
+  nn_layer_wave<NN_H, NN_OUTP, true>(L, Wb + nn_w_off(3), master + nn_b_off(3), lane);
+}
+
+// plain forward on normalized inputs X[B][NN_IN] -> Y[B][NN_OUT] (B % 64 == 0)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+    k_nn_forward(const float* __restrict__ Xg, float* __restrict__ Yg, const __bf16* __restrict__ Wb,
+                 const float* __restrict__ master) {
+  __shared__ __align__(16) FwdLds L;
+  const int lane = threadIdx.x;
+  const long s0 = (long)blockIdx.x * 64;
+  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? Xg[(s0 + lane) * NN_IN + k] : 0.0f);
+  __syncthreads();
+  nn_forward_wave(L, Wb, master, lane);
+  for (int o = 0; o < NN_OUT; ++o) Yg[(s0 + lane) * NN_OUT + o] = L.Y[lane][o];
+}
+
+// ---------------------------------------------------------------------------
+// Fused surrogate chemistry step (pre_input -> network -> post_predict,
+// reactions.jl:2-56): one wave = 64 interior cells.  The network runs for all
+// 64 lanes (uniform control flow around the MFMAs); each active lane then
+// applies four gates and either commits the prediction, with the FP64 energy
+// bookkeeping of the backward-Euler integrator (chem_epilogue), or appends its
+// cell to the fallback list and leaves it untouched:
+//   1. Mahalanobis distance of the normalized inputs <= d_max (out of distribution);
+//   2. element balance of the raw prediction (then projected to conserve elements);
+//   3. positivity, Y_k + dY_k >= -neg_tol;
+//   4. backward-Euler residual at the predicted state, max_k |R_k| / rho <= res_tol,
+//      R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T'), T' from the energy (error gate).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+    k_nn_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
+              const int64_t* __restrict__ tag, const DevParams* __restrict__ P,
+              const NNDev* __restrict__ C, const __bf16* __restrict__ Wb,
+              const float* __restrict__ master, Geom g, double dt, int32_t* __restrict__ fb_list,
+              int32_t* __restrict__ fb_count, int8_t* __restrict__ status) {
+  constexpr int NS = NN_OUT;
+  __shared__ __align__(16) FwdLds L;
+  const int lane = threadIdx.x;
+  const long ncell = (long)g.Nx * g.Ny * g.Nz;
+  const long id = (long)xcd_block() * 64 + lane;
+  const long NP = g.NP;
+  long p = 0;
+  bool act = false;
+  double r[NS], rho = 0.0, T = 0.0;
+  float xn[NN_IN];
+#pragma unroll
+  for (int k = 0; k < NN_IN; ++k) xn[k] = 0.0f;
+  if (id < ncell) {
+    const int a = (int)(id % g.Nx);
+    const int t = (int)(id / g.Nx);
+    const int b = t % g.Ny, c = t / g.Ny;
+    p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+    T = Q[p + 5 * NP];
+    act = (!tag || tag[p] == 0) && (T > P->T_criteria);
+    if (status) status[id] = 0;
+  }
+  if (act) {
+    // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
+    rho = Q[p];
+    const double rinv = 1.0 / fmax(rho, kEps);
+    const float lam = C->boxcox_lambda;
+    float x[NN_IN];
+    x[0] = (float)T;
+    x[1] = logf((float)Q[p + 4 * NP]);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      r[n] = rhoi[p + n * NP];
+      const float Y = (float)fmax(r[n] * rinv, 0.0);
+      x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
+    }
+#pragma unroll
+    for (int k = 0; k < NN_IN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? xn[k] : 0.0f);
+  __syncthreads();
+  nn_forward_wave(L, Wb, master, lane);
+  int code = act ? 1 : 0;  // 1 accepted, 2..5 the first gate that failed
+  if (act) {
+    // 1. Mahalanobis distance of the normalized inputs
+    float d2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NN_IN; ++i) {
+      float row = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NN_IN; ++j) row += C->mahal_inv[i][j] * (xn[j] - C->mahal_mean[j]);
+      d2 += (xn[i] - C->mahal_mean[i]) * row;
+    }
+    if (!(sqrtf(fmaxf(d2, 0.0f)) <= C->d_max)) code = 2;
+    // post_predict (reactions.jl:26-56) on the de-normalized outputs
+    const double rinv = 1.0 / fmax(rho, kEps);
+    const double lam = (double)C->boxcox_lambda;
+    double Y[NS], dY0[NS], dY[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      Y[n] = fmax(r[n] * rinv, 0.0);
+      const double o = (double)(L.Y[lane][n] * C->out_std[n] + C->out_mean[n]);
+      if (C->label_mode == 0) {
+        dY0[n] = o;
+      } else if (C->label_mode == 2) {
+        dY0[n] = (double)C->label_scale[n] * sinh(o);
+      } else {
+        const double base = lam * ((pow(Y[n], lam) - 1.0) / lam + o) + 1.0;
+        dY0[n] = (base > 0.0 ? pow(base, 1.0 / lam) : 0.0) - Y[n];
+      }
+    }
+    // 2. element balance of the raw prediction, then the projection
+    bool eok = true;
+    for (int e = 0; e < C->n_elem; ++e) {
+      double num = 0.0, den = 0.0;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const double ae = (double)C->elem[e][n] / P->mw[n];
+        num += ae * dY0[n];
+        den += ae * Y[n];
+      }
+      if (den > 0.0 && fabs(num) > (double)C->elem_tol * den) eok = false;
+    }
+    if (code == 1 && !eok) code = 3;
+    bool pok = true;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) v += C->proj[n][m] * dY0[m];
+      dY[n] = v;
+      // 3. positivity
+      if (Y[n] + dY[n] < -(double)C->neg_tol) pok = false;
+    }
+    if (code == 1 && !pok) code = 4;
+    if (code == 1) {
+      double rn[NS], dei = 0.0, sr = 0.0;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        rn[n] = fmax(Y[n] + dY[n], 0.0) * rho;
+        dei += -(P->lo[n][5] * P->Ru / P->mw[n]) * (rn[n] - r[n]);
+      }
+      // the state chem_epilogue would write, evaluated before committing
+      const double rhoc = fmax(rho, kEps);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) sr += rn[n];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) rn[n] *= rhoc / sr;
+      const double ein = Q[p + 6 * NP] + dei;
+      const double Tn = fmax(get_T<NS>(P, ein, rn, nullptr), kEps);
+      // 4. backward-Euler residual at (rho', T')
+      if (C->res_tol > 0.0f) {
+        double sc[NS], wdot[NS], Adum[NS][NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
+        production_rate<NS, false>(P, sc, Tn, Adum, wdot);
+        double rmax = 0.0;
+#pragma unroll
+        for (int n = 0; n < NS; ++n)
+          rmax = fmax(rmax, fabs(rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt));
+        if (!(rmax <= (double)C->res_tol * rhoc)) code = 5;
+      }
+      if (code == 1) {
+        U[p + 4 * NP] += dei;
+        Q[p + 4 * NP] = fmax(pmixture<NS>(P, Tn, rn), kEps);
+        Q[p + 5 * NP] = Tn;
+        Q[p + 6 * NP] = ein;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = rn[n];
+      }
+    }
+    if (status) status[id] = (int8_t)code;
+  }
+  const bool ok = (code == 1);
+  // wave-aggregated append of the rejected cells
+  const bool rej = act && !ok;  // code 2..5
+  const unsigned long long mask = __ballot(rej);
+  if (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (rej) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)id;
+  }
+}
+
+// P = I - S E^T (E S E^T)^-1 E with E[e][k] = atoms_ek / W_k and S = diag(out_std^2):
+// the out_std-weighted orthogonal projection onto {dY : E dY = 0}, which
+// conserves every element (and so total mass).  Returns nonzero if E S E^T is
+// singular.
+static int nn_projection(const f3d_ctx* c, const f3d_nn_config* cfg, double (&Pm)[NN_OUT][NN_OUT]) {
+  const int ne = cfg->n_elem;
+  for (int i = 0; i < NN_OUT; ++i)
+    for (int j = 0; j < NN_OUT; ++j) Pm[i][j] = (i == j) ? 1.0 : 0.0;
+  if (ne == 0) return 0;
+  double E[4][NN_OUT], S[NN_OUT], G[4][8];
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < NN_OUT; ++k) E[e][k] = cfg->elem[e][k] / c->host_params.mw[k];
+  for (int k = 0; k < NN_OUT; ++k) S[k] = (double)cfg->out_std[k] * cfg->out_std[k];
+  // G = [E S E^T | I], Gauss-Jordan with partial pivoting -> inverse in G[:, ne:]
+  for (int a = 0; a < ne; ++a)
+    for (int b = 0; b < 2 * ne; ++b) {
+      double v = 0.0;
+      if (b < ne)
+        for (int k = 0; k < NN_OUT; ++k) v += E[a][k] * S[k] * E[b][k];
+      else
+        v = (b - ne == a) ? 1.0 : 0.0;
+      G[a][b] = v;
+    }
+  double scale = 0.0;
+  for (int a = 0; a < ne; ++a) scale = std::fmax(scale, std::fabs(G[a][a]));
+  for (int col = 0; col < ne; ++col) {
+    int piv = col;
+    for (int a = col + 1; a < ne; ++a)
+      if (std::fabs(G[a][col]) > std::fabs(G[piv][col])) piv = a;
+    if (!(std::fabs(G[piv][col]) > 1e-14 * scale)) return 1;
+    for (int b = 0; b < 2 * ne; ++b) std::swap(G[col][b], G[piv][b]);
+    const double d = G[col][col];
+    for (int b = 0; b < 2 * ne; ++b) G[col][b] /= d;
+    for (int a = 0; a < ne; ++a)
+      if (a != col) {
+        const double f = G[a][col];
+        for (int b = 0; b < 2 * ne; ++b) G[a][b] -= f * G[col][b];
+      }
+  }
+  // Pm -= S E^T Ginv E
+  for (int i = 0; i < NN_OUT; ++i)
+    for (int j = 0; j < NN_OUT; ++j) {
+      double v = 0.0;
+      for (int a = 0; a < ne; ++a)
+        for (int b = 0; b < ne; ++b) v += S[i] * E[a][i] * G[a][ne + b] * E[b][j];
+      Pm[i][j] -= v;
+    }
+  return 0;
+}
+
+static int nn_check(f3d_ctx* c) {
+  if (!c->nn_ready) {
+    set_error("call f3d_nn_init first");
+    return F3D_ERR_STATE;
+  }
+  return F3D_OK;
+}
+
+static int nn_partials(f3d_ctx* c, long tiles) {
+  if (tiles <= c->nn_partial_tiles) return F3D_OK;
+  if (c->nn_partial) (void)hipFree(c->nn_partial);
+  c->nn_partial = nullptr;
+  c->nn_partial_tiles = 0;
+  if (hipMalloc(&c->nn_partial, sizeof(float) * tiles * (NN_NPARAM + 1)) != hipSuccess) {
+    set_error("f3d_nn: partial-gradient allocation failed");
+    return F3D_ERR_ALLOC;
+  }
+  c->nn_partial_tiles = tiles;
+  return F3D_OK;
+}
+
+void nn_free(f3d_ctx* c) {
+  void* ptrs[] = {c->nn_master, c->nn_m1, c->nn_m2, c->nn_Wb, c->nn_Wtb, c->nn_partial,
+                  c->nn_cfg, c->nn_fb, c->nn_fbcount};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  c->nn_ready = false;
+}
+
+}  // namespace f3d
+
+using namespace f3d;
+
+extern "C" {
+
+int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
+  if (!c || !cfg) {
+    set_error("f3d_nn_init: null argument");
+    return F3D_ERR_ARG;
+  }
+  if (c->g.Ns != NN_OUT) {
+    set_error("f3d_nn_init: the surrogate predicts 9 species (context Ns must be 9)");
+    return F3D_ERR_UNSUPPORTED;
+  }
+  if (cfg->n_elem < 0 || cfg->n_elem > 4 || !(cfg->dt > 0) || !(cfg->boxcox_lambda > 0) ||
+      cfg->label_mode < 0 || cfg->label_mode > 2) {
+    set_error("f3d_nn_init: need 0 <= n_elem <= 4, dt > 0, boxcox_lambda > 0, label_mode 0..2");
+    return F3D_ERR_ARG;
+  }
+  for (int k = 0; k < NN_IN; ++k)
+    if (!(cfg->in_std[k] > 0)) {
+      set_error("f3d_nn_init: in_std must be positive");
+      return F3D_ERR_ARG;
+    }
+  NNDev h;
+  std::memset(&h, 0, sizeof(h));
+  for (int k = 0; k < NN_IN; ++k) {
+    h.in_mean[k] = cfg->in_mean[k];
+    h.in_std[k] = cfg->in_std[k];
+    h.mahal_mean[k] = cfg->mahal_mean[k];
+    for (int j = 0; j < NN_IN; ++j) h.mahal_inv[k][j] = cfg->mahal_inv[k * NN_IN + j];
+  }
+  for (int k = 0; k < NN_OUT; ++k) {
+    h.out_mean[k] = cfg->out_mean[k];
+    h.out_std[k] = cfg->out_std[k];
+  }
+  h.d_max = cfg->d_max;
+  h.elem_tol = cfg->elem_tol;
+  h.neg_tol = cfg->neg_tol;
+  h.res_tol = cfg->res_tol;
+  h.boxcox_lambda = cfg->boxcox_lambda;
+  h.label_mode = cfg->label_mode;
+  for (int k = 0; k < NN_OUT; ++k) h.label_scale[k] = cfg->label_scale[k];
+  h.n_elem = cfg->n_elem;
+  for (int e = 0; e < 4; ++e)
+    for (int k = 0; k < NN_OUT; ++k) h.elem[e][k] = cfg->elem[e][k];
+  h.dt = cfg->dt;
+  if (nn_projection(c, cfg, h.proj)) {
+    set_error("f3d_nn_init: element matrix is rank deficient (an element with no species?)");
+    return F3D_ERR_ARG;
+  }
+  if (!c->nn_ready) {
+    const long ncell = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+    bool okm = hipMalloc(&c->nn_master, sizeof(float) * NN_NPARAM) == hipSuccess &&
+               hipMalloc(&c->nn_m1, sizeof(float) * NN_NPARAM) == hipSuccess &&
+               hipMalloc(&c->nn_m2, sizeof(float) * NN_NPARAM) == hipSuccess &&
+               hipMalloc(&c->nn_Wb, 2 * NN_NPARAM) == hipSuccess &&
+               hipMalloc(&c->nn_Wtb, 2 * NN_NPARAM) == hipSuccess &&
+               hipMalloc(&c->nn_cfg, sizeof(NNDev)) == hipSuccess &&
+               hipMalloc(&c->nn_fb, sizeof(int32_t) * ncell) == hipSuccess &&
+               hipMalloc(&c->nn_fbcount, sizeof(int32_t)) == hipSuccess;
+    if (!okm) {
+      nn_free(c);
+      set_error("f3d_nn_init: device allocation failed");
+      return F3D_ERR_ALLOC;
+    }
+    (void)hipMemset(c->nn_master, 0, sizeof(float) * NN_NPARAM);
+    (void)hipMemset(c->nn_m1, 0, sizeof(float) * NN_NPARAM);
+    (void)hipMemset(c->nn_m2, 0, sizeof(float) * NN_NPARAM);
+    (void)hipMemset(c->nn_Wb, 0, 2 * NN_NPARAM);
+    (void)hipMemset(c->nn_Wtb, 0, 2 * NN_NPARAM);
+    c->nn_ready = true;
+  }
+  c->nn_dt = cfg->dt;
+  return hip_check(hipMemcpy(c->nn_cfg, &h, sizeof(NNDev), hipMemcpyHostToDevice), "f3d_nn_init");
+}
+
+int f3d_nn_set_params(f3d_ctx* c, const float* params) {
+  if (!c || !params) {
+    set_error("f3d_nn_set_params: null argument");
+    return F3D_ERR_ARG;
+  }
+  int rc = nn_check(c);
+  if (rc) return rc;
+  rc = hip_check(hipMemcpy(c->nn_master, params, sizeof(float) * NN_NPARAM, hipMemcpyDefault),
+                 "f3d_nn_set_params");
+  if (rc) return rc;
+  (void)hipMemset(c->nn_m1, 0, sizeof(float) * NN_NPARAM);
+  (void)hipMemset(c->nn_m2, 0, sizeof(float) * NN_NPARAM);
+  c->nn_step = 0;
+  hipLaunchKernelGGL(k_nn_cast, dim3(nblocks(NN_NPARAM)), dim3(kBlock), 0, 0, c->nn_master,
+                     (__bf16*)c->nn_Wb, (__bf16*)c->nn_Wtb);
+  rc = hip_check(hipGetLastError(), "f3d_nn_set_params");
+  if (rc) return rc;
+  return hip_check(hipDeviceSynchronize(), "f3d_nn_set_params");
+}
+
+int f3d_nn_get_params(f3d_ctx* c, float* params) {
+  if (!c || !params) {
+    set_error("f3d_nn_get_params: null argument");
+    return F3D_ERR_ARG;
+  }
+  int rc = nn_check(c);
+  if (rc) return rc;
+  return hip_check(hipMemcpy(params, c->nn_master, sizeof(float) * NN_NPARAM, hipMemcpyDefault),
+                   "f3d_nn_get_params");
+}
+
+static int nn_batch_args(f3d_ctx* c, const void* X, const void* Y, int32_t B, const char* who) {
+  if (!c || !X || !Y) {
+    set_error(std::string(who) + ": null argument");
+    return F3D_ERR_ARG;
+  }
+  if (B <= 0 || B % 64 != 0) {
+    set_error(std::string(who) + ": batch size must be a positive multiple of 64");
+    return F3D_ERR_ARG;
+  }
+  return nn_check(c);
+}
+
+int f3d_nn_grad(f3d_ctx* c, const float* X, const float* Y, int32_t B, float* grad, hipStream_t s) {
+  int rc = nn_batch_args(c, X, Y, B, "f3d_nn_grad");
+  if (rc) return rc;
+  if (!grad) {
+    set_error("f3d_nn_grad: null grad");
+    return F3D_ERR_ARG;
+  }
+  const long tiles = B / 64;
+  rc = nn_partials(c, tiles);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B,
+                     (const __bf16*)c->nn_Wb, (const __bf16*)c->nn_Wtb, c->nn_master,
+                     c->nn_partial);
+  hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(NN_NPARAM + 1)), dim3(kBlock), 0, s,
+                     c->nn_partial, (int)tiles, grad);
+  return hip_check(hipGetLastError(), "f3d_nn_grad");
+}
+
+int f3d_nn_train_step(f3d_ctx* c, const float* X, const float* Y, int32_t B, float lr,
+                      float* loss, hipStream_t s) {
+  int rc = nn_batch_args(c, X, Y, B, "f3d_nn_train_step");
+  if (rc) return rc;
+  const long tiles = B / 64;
+  rc = nn_partials(c, tiles);
+  if (rc) return rc;
+  c->nn_step += 1;
+  const double b1 = 0.9, b2 = 0.999;
+  const float bc1 = (float)(1.0 - std::pow(b1, (double)c->nn_step));
+  const float bc2 = (float)(1.0 - std::pow(b2, (double)c->nn_step));
+  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B,
+                     (const __bf16*)c->nn_Wb, (const __bf16*)c->nn_Wtb, c->nn_master,
+                     c->nn_partial);
+  hipLaunchKernelGGL(k_nn_adam, dim3(nblocks(NN_NPARAM + 1)), dim3(kBlock), 0, s, c->nn_partial,
+                     (int)tiles, c->nn_master, c->nn_m1, c->nn_m2, (__bf16*)c->nn_Wb,
+                     (__bf16*)c->nn_Wtb, lr, (float)b1, (float)b2, 1e-8f, bc1, bc2, loss);
+  return hip_check(hipGetLastError(), "f3d_nn_train_step");
+}
+
+int f3d_nn_forward(f3d_ctx* c, const float* X, float* Y, int32_t B, hipStream_t s) {
+  int rc = nn_batch_args(c, X, Y, B, "f3d_nn_forward");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_nn_forward, dim3(B / 64), dim3(64), 0, s, X, Y, (const __bf16*)c->nn_Wb,
+                     c->nn_master);
+  return hip_check(hipGetLastError(), "f3d_nn_forward");
+}
+
+int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32_t maxiter,
+                double rtol, double ttol, int8_t* status, int64_t* n_fallback, const int64_t* tag,
+                hipStream_t s) {
+  if (!c || !U || !Q || !rhoi || !n_fallback) {
+    set_error("f3d_nn_chem: null argument");
+    return F3D_ERR_ARG;
+  }
+  int rc = nn_check(c);
+  if (rc) return rc;
+  if (std::fabs(dt - c->nn_dt) > 1e-12 * c->nn_dt) {
+    set_error("f3d_nn_chem: dt differs from the step the network was trained for");
+    return F3D_ERR_ARG;
+  }
+  if (maxiter < 1 || maxiter > 1000 || !(rtol > 0) || !(ttol > 0)) {
+    set_error("f3d_nn_chem: need 1 <= maxiter <= 1000, rtol > 0, ttol > 0");
+    return F3D_ERR_ARG;
+  }
+  const long ncell = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+  rc = hip_check(hipMemsetAsync(c->nn_fbcount, 0, sizeof(int32_t), s), "f3d_nn_chem");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_nn_chem, dim3((unsigned)((ncell + 63) / 64)), dim3(64), 0, s, U, Q, rhoi,
+                     tag, c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb,
+                     c->nn_master, c->g, dt, c->nn_fb, c->nn_fbcount, status);
+  rc = hip_check(hipGetLastError(), "f3d_nn_chem");
+  if (rc) return rc;
+  int32_t nfb = 0;
+  rc = hip_check(hipMemcpyAsync(&nfb, c->nn_fbcount, sizeof(int32_t), hipMemcpyDeviceToHost, s),
+                 "f3d_nn_chem");
+  if (rc) return rc;
+  rc = hip_check(hipStreamSynchronize(s), "f3d_nn_chem");
+  if (rc) return rc;
+  *n_fallback = nfb;
+  if (nfb > 0)
+    return launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fb, nfb);
+  return F3D_OK;
+}
+
+}  // extern "C"

@@ -81,6 +81,8 @@ class Mechanism:
     loP: np.ndarray            # [3, Nr]
     Troe: np.ndarray           # [4, Nr]
     equations: List[str] = dataclasses.field(default_factory=list)
+    elements: List[str] = dataclasses.field(default_factory=list)
+    composition: Optional[np.ndarray] = None  # [n_elements, Ns] atoms per molecule
 
     @property
     def Ns(self) -> int:
@@ -98,6 +100,8 @@ class Mechanism:
         d = {}
         for f in dataclasses.fields(self):
             v = getattr(self, f.name)
+            if v is None:
+                continue
             if isinstance(v, np.ndarray):
                 d[f.name] = {"shape": list(v.shape), "dtype": str(v.dtype),
                              "data": v.flatten(order="F").tolist()}
@@ -110,6 +114,8 @@ class Mechanism:
         d = json.loads(text)
         kw = {}
         for f in dataclasses.fields(Mechanism):
+            if f.name not in d:
+                continue
             v = d[f.name]
             if isinstance(v, dict) and "shape" in v:
                 kw[f.name] = np.array(v["data"], dtype=v["dtype"]).reshape(
@@ -288,8 +294,14 @@ def load_yaml(path: str, fit: bool = True) -> Mechanism:
             raise ValueError(f"Not supported reaction type of reaction {j + 1}")
         sgm[j] = int((vf[:, j] - vr[:, j]).sum())
 
+    elements = [e for e in phase.get("elements", []) if any(
+        e in sp_by_name[n]["composition"] for n in names)]
+    comp = np.zeros((len(elements), Ns))
+    for n, name in enumerate(names):
+        for el, cnt in sp_by_name[name]["composition"].items():
+            comp[elements.index(el), n] = cnt
     mech = Mechanism(
-        species=names, Ru=GAS_CONSTANT * 1e-3, min_temp=float(max(tmins)),
+        species=names, Ru=GAS_CONSTANT * 1e-3, elements=elements, composition=comp, min_temp=float(max(tmins)),
         max_temp=float(min(tmaxs)), mw=mw, coeffs_sep=sep, coeffs_lo=lo,
         coeffs_hi=hi, visc_poly=np.zeros((5, Ns)), conduct_poly=np.zeros((5, Ns)),
         binarydiff_poly=np.zeros((5, Ns, Ns)), atm=ONE_ATM, reaction_type=rtype,

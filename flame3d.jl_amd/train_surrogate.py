@@ -1,0 +1,135 @@
+"""Train the H2/O2 chemistry surrogate on the GPU and save it.
+
+    python flame3d.jl_amd/train_surrogate.py --out PATH [--samples-shape 64 64 32]
+        [--steps 2000] [--records 32] [--epochs 60] [--batch 8192] [--lr 2e-3]
+
+Data: trajectories of the library's FP64 backward-Euler Newton integrator
+(f3d_chem_newton, dt = 1e-7 s) from H2/O2/N2 reactors with T ~ U[900, 2500] K,
+P ~ U[1, 10] atm, mixture fraction ~ Beta(2, 2) (NN/H2/gen_data.py pattern).
+Training: the library's BF16 MFMA forward/backward and Adam kernels
+(f3d_nn_train_step).  Then a held-out check on fresh reactors with a
+different seed: surrogate step (with gates and Newton fallback) against the
+pure Newton step.  Prints one JSON summary line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load_package():
+    if "flame3d_amd" in sys.modules:
+        return sys.modules["flame3d_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "flame3d_amd", os.path.join(HERE, "__init__.py"), submodule_search_locations=[HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["flame3d_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def evaluate(pkg, mech, path, shape=(64, 64, 64), seed=777, log=print, **gates):
+    """Surrogate step vs pure Newton step on fresh reactors (held-out seed)."""
+    import torch
+    abi, cases, sur = pkg.abi, pkg.cases, pkg.surrogate
+    Nx, Ny, Nz = shape
+    ctx = abi.Context(Nx, Ny, Nz, mech, precision="fp64")
+    sur.attach(ctx, path, **gates)
+    Q, rhoi = cases.chem_cells(shape, mech, seed=seed, dev="cuda")
+    U = torch.zeros((5,) + tuple(Q.shape[1:]), dtype=torch.float64, device="cuda")
+    Qn, rn, Un = Q.clone(), rhoi.clone(), U.clone()
+    iters = torch.zeros((Nz, Ny, Nx), dtype=torch.int32, device="cuda")
+    ctx.chem_newton(Un, Qn, rn, sur.DT, 50, 1e-10, 1e-6, iters=iters)
+    status = torch.zeros((Nz, Ny, Nx), dtype=torch.int8, device="cuda")
+    nfb = ctx.nn_chem(U, Q, rhoi, sur.DT, 50, 1e-10, 1e-6, status=status)
+    torch.cuda.synchronize()
+    Y1, Y0 = sur.mass_fractions(Q, rhoi), sur.mass_fractions(Qn, rn)
+    T1, T0 = sur.interior(Q)[:, 5], sur.interior(Qn)[:, 5]
+    dY = (Y1 - Y0).abs()
+    dT = (T1 - T0).abs()
+    acc = (status.reshape(-1) == 1)
+    it = iters.reshape(-1).double()
+    res = {
+        "cells": Nx * Ny * Nz,
+        "fallback_fraction": nfb / (Nx * Ny * Nz),
+        "Y_linf": float(dY.max()), "Y_l2": float(torch.sqrt((dY ** 2).mean())),
+        "T_linf": float(dT.max()), "T_l2": float(torch.sqrt((dT ** 2).mean())),
+        "newton_iters_mean": float(it.mean()),
+        "newton_iters_avoided_fraction": float(it[acc].sum() / it.sum()),
+    }
+    ctx.close()
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--samples-shape", type=int, nargs=3, default=(64, 64, 32))
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--records", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--decay-every", type=int, default=20)
+    ap.add_argument("--res-tol", type=float, default=2e-6)
+    ap.add_argument("--seed", type=int, default=12345)
+    a = ap.parse_args(argv)
+
+    pkg = _load_package()
+    import flame3d_amd.abi  # noqa: F401
+    import flame3d_amd.cases  # noqa: F401
+    import flame3d_amd.mech as mm
+    import flame3d_amd.surrogate as sur
+    import torch
+
+    def log(msg):
+        print(msg, flush=True)
+
+    mech = mm.load("h2_lidryer")
+    t0 = time.time()
+    log(f"generating data: {a.samples_shape} reactors x {a.records} records over {a.steps} steps")
+    data = sur.generate(mech, tuple(a.samples_shape), a.steps, a.records, a.seed, log=log)
+    t_gen = time.time() - t0
+    M = data.X.shape[0]
+    log(f"  {M} samples in {t_gen:.1f}s, mean Newton iterations {data.newton_iters:.2f}")
+    cfg = sur.make_config(mech, data, res_tol=a.res_tol)
+    Xn, Yn = sur.normalize(cfg, data)
+    del data
+    # hold out 1/16 for validation
+    g = torch.Generator(device="cuda").manual_seed(a.seed)
+    perm = torch.randperm(M, generator=g, device="cuda")
+    nval = (M // 16 // 64) * 64
+    Xv, Yv = Xn[perm[:nval]].contiguous(), Yn[perm[:nval]].contiguous()
+    Xt, Yt = Xn[perm[nval:]].contiguous(), Yn[perm[nval:]].contiguous()
+    del Xn, Yn
+    ctx = pkg.abi.Context(8, 8, 8, mech, precision="fp64")
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(sur.init_params(a.seed))
+    t1 = time.time()
+    hist = sur.train(ctx, Xt, Yt, a.epochs, a.batch, a.lr, 0.3, a.decay_every, a.seed, log=log)
+    torch.cuda.synchronize()
+    t_train = time.time() - t1
+    val = sur.mse(ctx, Xv, Yv)
+    params = ctx.nn_get_params()
+    meta = {"samples": M, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+            "final_train_loss": hist[-1], "val_mse": val, "train_seconds": t_train,
+            "generate_seconds": t_gen, "shape": list(a.samples_shape), "steps": a.steps,
+            "records": a.records, "seed": a.seed}
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    sur.save(a.out, cfg, params, meta)
+    ctx.close()
+    log(f"saved {a.out}: val mse {val:.3e}, d_max {cfg.d_max:.3f}")
+    ev = evaluate(pkg, mech, a.out, log=log)
+    meta["heldout"] = ev
+    sur.save(a.out, cfg, params, meta)
+    print(json.dumps(meta), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -165,12 +165,72 @@ int f3d_chem_stiff(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
 int f3d_chem_explicit(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
                       const int64_t* tag, hipStream_t s);
 /* Backward-Euler Newton on (rho_1..rho_Ns, T) with a dense (Ns+1)x(Ns+1)
- * pivoted LU per cell; iters [Nx*Ny*Nz] (device, may be NULL) receives the
- * Newton iteration count of each interior cell.  No reference counterpart
- * (extends eval_gpu_stiff, whose first iterate it reproduces). */
+ * pivoted LU per cell; maxiter bounds the iterations of one solve.  A cell
+ * whose iterate leaves the admissible set (non-finite, T outside
+ * [min_temp/2, 1.5 max_temp], rho_k < -rho) or does not converge restarts
+ * from its initial state with 2, 4, .., 64 backward-Euler substeps of dt.
+ * iters [Nx*Ny*Nz] (device, may be NULL) receives the total iteration count
+ * of each interior cell, negated when every attempt failed (that cell is
+ * then left unreacted).  No reference counterpart (extends eval_gpu_stiff,
+ * whose first iterate it reproduces; run.jl:23 sub-cycles chemistry too). */
 int f3d_chem_newton(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
                     int32_t maxiter, double rtol, double ttol, int32_t* iters,
                     const int64_t* tag, hipStream_t s);
+
+/* ---- learned chemistry surrogate (BF16 MFMA MLP 11-64-64-64-9, GELU) -----
+ * Reference pattern: pre_input / evalModel / post_predict, reactions.jl:2-72
+ * (Box-Cox inputs, z-scored inputs and labels, GEMM + GELU layers).  Inputs
+ * are (T, log P, BoxCox(Y_1..Y_9)), outputs dY_1..dY_9 over the step dt; the
+ * network is trained here, on the GPU, by f3d_nn_train_step. */
+#define F3D_NN_NPARAM 12512 /* packed FP32 parameters, layout in csrc/f3d_nn.h */
+typedef struct {
+  float in_mean[11], in_std[11], out_mean[9], out_std[9];
+  float mahal_mean[11];
+  float mahal_inv[121];  /* inverse covariance of the normalized inputs, row-major */
+  float d_max;           /* cells with Mahalanobis distance > d_max fall back */
+  float elem_tol;        /* |element balance of dY| / element content > elem_tol falls back */
+  float neg_tol;         /* Y_k + dY_k < -neg_tol falls back */
+  float res_tol;         /* backward-Euler residual max_k |R_k|/rho > res_tol falls back
+                            (R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T')); <= 0: off */
+  float boxcox_lambda;
+  int32_t label_mode;    /* outputs: 0 dY_k; 1 d BoxCox(Y_k) (post_predict form,
+                            Y' = (lambda (BoxCox(Y) + out) + 1)^(1/lambda));
+                            2 asinh(dY_k / label_scale_k) */
+  float label_scale[9];  /* label_mode 2: dY_k = label_scale_k sinh(out_k) */
+  int32_t n_elem;
+  float elem[4][9];      /* atoms of element e in species k */
+  double dt;             /* chemistry step the network predicts */
+} f3d_nn_config;
+/* Allocates the network (parameters zero) and stores the configuration;
+ * the context must have Ns = 9.  May be called again to change the config.
+ * With n_elem > 0 every prediction is projected onto the element-conserving
+ * subspace (weighted by out_std^2) after the element-balance gate. */
+int f3d_nn_init(f3d_ctx* ctx, const f3d_nn_config* cfg);
+/* Host FP32 parameter vector of F3D_NN_NPARAM floats; set resets Adam. */
+int f3d_nn_set_params(f3d_ctx* ctx, const float* params);
+int f3d_nn_get_params(f3d_ctx* ctx, float* params);
+/* One Adam step on the MSE of normalized outputs for a device batch
+ * X[B][11] (normalized inputs), Y[B][9] (normalized targets), B % 64 == 0;
+ * loss (device float, may be NULL) receives sum (y - t)^2 / (B * 9). */
+int f3d_nn_train_step(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float lr,
+                      float* loss, hipStream_t s);
+/* Gradient of that loss w.r.t. the packed parameters (device, F3D_NN_NPARAM
+ * floats, followed by the loss) without updating the weights. */
+int f3d_nn_grad(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float* grad,
+                hipStream_t s);
+/* Forward on normalized inputs: Y[B][9] = net(X[B][11]), B % 64 == 0. */
+int f3d_nn_forward(f3d_ctx* ctx, const float* X, float* Y, int32_t B, hipStream_t s);
+/* Surrogate chemistry step of dt (= cfg.dt) over the interior: cells with
+ * T > T_criteria are predicted by the network; cells failing the gates are
+ * integrated by the FP64 backward-Euler Newton (f3d_chem_newton settings).
+ * status [Nx*Ny*Nz] (device, may be NULL): 0 untouched, 1 surrogate; Newton
+ * fallback after the gate that rejected the cell: 2 Mahalanobis distance,
+ * 3 element balance, 4 positivity, 5 backward-Euler residual.
+ * *n_fallback (host) receives the number of fallback cells; this call
+ * synchronizes the stream to size the fallback launch. */
+int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int32_t maxiter,
+                double rtol, double ttol, int8_t* status, int64_t* n_fallback, const int64_t* tag,
+                hipStream_t s);
 
 /* ---- stage level -------------------------------------------------------- */
 /* Registers the block's metrics (9 device pointers, order as above) and J;

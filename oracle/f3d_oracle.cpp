@@ -24,6 +24,7 @@
 namespace {
 
 constexpr double kEps = 2.220446049250313e-16;  // eps(Float64)
+constexpr int kNewtonMaxSub = 64;  // most backward-Euler substeps the Newton mode tries
 
 struct Grid {
   int Nx, Ny, Nz, NG, Ns, Nr;
@@ -1132,6 +1133,58 @@ void or_chem_explicit(const or_dims* d, double* U, double* Q, double* rhoi, doub
 // dR_T/drho_l = (e_l(T) + a6_lo,l) Ru/W_l, dR_T/dT = CV(T, rho).  The first
 // iterate from (rho0, T0) reproduces the point-implicit update.  Converged
 // when max_k |d rho_k| <= rtol * rho and |dT| <= ttol; iters = updates taken.
+// One backward-Euler solve over h of the reactor (rho_k, T) starting at
+// (r, T) = (rs, Ts), with the total energy fixed at E0: Newton on
+//   F_k = rho_k - rs_k - h W_k wdot_k(rho, T) = 0,  F_E = E(rho, T) - E0 = 0
+// with the exact mass-action Jacobian (+ third-body terms) and a forward
+// difference in T.  Returns false when the iterate leaves the admissible
+// set (non-finite, T outside [T_lo, T_hi], rho_k < -rho_tot) or maxiter
+// iterations pass without convergence; *it counts the iterations taken.
+static bool be_solve(const Grid& g, const or_dims* d, const or_thermo* th,
+                     const or_kinetics* kin, double* r, double& T, double h, double E0,
+                     const double* hf, double rho_tot, double T_lo, double T_hi, int maxiter,
+                     double rtol, double ttol, int& it) {
+  const int Ns = g.Ns;
+  const int N1 = Ns + 1;
+  double rs[64], s[64], wdot[64], wdot2[64], eis[64], rhs[64], dx[64];
+  std::vector<double> Arate(Ns * Ns), A(N1 * N1);
+  for (int n = 0; n < Ns; ++n) rs[n] = r[n];
+  for (int k = 0; k < maxiter; ++k) {
+    for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
+    or_production_rate(d, th, kin, s, T, kEps, 1, wdot, Arate.data());
+    double hT = 1e-6 * T;
+    or_production_rate(d, th, kin, s, T + hT, kEps, 1, wdot2, nullptr);
+    for (int n = 0; n < Ns; ++n) {
+      double src = wdot[n] * th->mw[n] * 1e6 * h;
+      rhs[n] = -(r[n] - rs[n] - src);
+      for (int l = 0; l < Ns; ++l)
+        A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * h;
+      A[n + N1 * Ns] = -((wdot2[n] - wdot[n]) / hT) * th->mw[n] * 1e6 * h;
+    }
+    double E = or_internal_energy(d, th, T, r);
+    for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
+    rhs[Ns] = -(E - E0);
+    ei_specs(g, th, T, eis);
+    for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * th->Ru / th->mw[l] + hf[l];
+    A[Ns + N1 * Ns] = or_cv(d, th, T, r);
+    or_gauss_solve(N1, A.data(), rhs, dx);
+    double rho = 0;
+    for (int n = 0; n < Ns; ++n) rho += r[n];
+    double mx = 0;
+    bool bad = false;
+    for (int n = 0; n < Ns; ++n) {
+      r[n] += dx[n];
+      mx = std::max(mx, std::fabs(dx[n]));
+      bad = bad || !(r[n] > -rho_tot);
+    }
+    T += dx[Ns];
+    ++it;
+    if (bad || !(T > T_lo && T < T_hi)) return false;
+    if (mx <= rtol * rho && std::fabs(dx[Ns]) <= ttol) return true;
+  }
+  return false;
+}
+
 void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double dt,
                     const or_thermo* th, const or_kinetics* kin, const or_scheme* sc,
                     const int64_t* tag, int32_t maxiter, double rtol, double ttol,
@@ -1139,7 +1192,7 @@ void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double
   Grid g(d);
   const long NP = g.NP;
   const int Ns = g.Ns;
-  const int N1 = Ns + 1;
+  const double T_lo = 0.5 * th->min_temp, T_hi = 1.5 * th->max_temp;
 #pragma omp parallel for schedule(dynamic, 64)
   for (long c = 0; c < g.Nz; ++c)
     for (long b = 0; b < g.Ny; ++b)
@@ -1150,48 +1203,32 @@ void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double
         if (not_fluid(tag, p)) continue;
         double T0 = Q[p + 5 * NP];
         if (!(T0 > sc->T_criteria)) continue;
-        double r0[64], r[64], s[64], wdot[64], wdot2[64], eis[64], rhs[64], dx[64];
-        std::vector<double> Arate(Ns * Ns), A(N1 * N1);
-        double hf[64];
+        double r0[64], r[64], hf[64];
+        double rho_tot = 0;
         for (int n = 0; n < Ns; ++n) {
           r0[n] = rhoi[p + n * NP];
-          r[n] = r0[n];
           hf[n] = th->coeffs_lo[5 + 7 * n] * th->Ru / th->mw[n];
+          rho_tot += r0[n];
         }
         double E0 = or_internal_energy(d, th, T0, r0);
         for (int n = 0; n < Ns; ++n) E0 += hf[n] * r0[n];
-        double T = T0;
+        // one step of dt; on failure restart from the initial state with
+        // twice as many backward-Euler substeps (at most kNewtonMaxSub)
         int it = 0;
-        for (; it < maxiter;) {
-          for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
-          or_production_rate(d, th, kin, s, T, kEps, 1, wdot, Arate.data());
-          double hT = 1e-6 * T;
-          or_production_rate(d, th, kin, s, T + hT, kEps, 1, wdot2, nullptr);
-          // residual (negated) and Jacobian
-          for (int n = 0; n < Ns; ++n) {
-            double src = wdot[n] * th->mw[n] * 1e6 * dt;
-            rhs[n] = -(r[n] - r0[n] - src);
-            for (int l = 0; l < Ns; ++l)
-              A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * dt;
-            A[n + N1 * Ns] = -((wdot2[n] - wdot[n]) / hT) * th->mw[n] * 1e6 * dt;
-          }
-          double E = or_internal_energy(d, th, T, r);
-          for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
-          rhs[Ns] = -(E - E0);
-          ei_specs(g, th, T, eis);
-          for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * th->Ru / th->mw[l] + hf[l];
-          A[Ns + N1 * Ns] = or_cv(d, th, T, r);
-          or_gauss_solve(N1, A.data(), rhs, dx);
-          double rho = 0;
-          for (int n = 0; n < Ns; ++n) rho += r[n];
-          double mx = 0;
-          for (int n = 0; n < Ns; ++n) {
-            r[n] += dx[n];
-            mx = std::max(mx, std::fabs(dx[n]));
-          }
-          T += dx[Ns];
-          ++it;
-          if (mx <= rtol * rho && std::fabs(dx[Ns]) <= ttol) break;
+        bool ok = false;
+        double T = T0;
+        for (int nsub = 1; nsub <= kNewtonMaxSub && !ok; nsub *= 2) {
+          for (int n = 0; n < Ns; ++n) r[n] = r0[n];
+          T = T0;
+          ok = true;
+          const double h = dt / nsub;
+          for (int k = 0; k < nsub && ok; ++k)
+            ok = be_solve(g, d, th, kin, r, T, h, E0, hf, rho_tot, T_lo, T_hi, maxiter, rtol,
+                          ttol, it);
+        }
+        if (!ok) {  // no admissible solution: the reactor is left unreacted
+          for (int n = 0; n < Ns; ++n) r[n] = r0[n];
+          it = -it;
         }
         if (iters) iters[cell] = it;
         // energy bookkeeping as eval_gpu_stiff: U[5] and e move by the

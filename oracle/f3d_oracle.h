@@ -101,7 +101,10 @@ void or_chem_stiff(const or_dims* d, double* U, double* Q, double* rhoi, double 
 void or_chem_explicit(const or_dims* d, double* U, double* Q, double* rhoi, double dt,
                       const or_thermo* th, const or_kinetics* kin, const or_scheme* sc,
                       const int64_t* tag);
-/* backward-Euler Newton on (rho_1..rho_Ns, T) with a dense (Ns+1) LU */
+/* backward-Euler Newton on (rho_1..rho_Ns, T) with a dense (Ns+1) LU; a
+ * cell whose iterate leaves the admissible set restarts with 2, 4, .. 64
+ * substeps; iters = total iterations (negated if every attempt failed, the
+ * cell then left unreacted) */
 void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double dt,
                     const or_thermo* th, const or_kinetics* kin, const or_scheme* sc,
                     const int64_t* tag, int32_t maxiter, double rtol, double ttol,

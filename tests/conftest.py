@@ -27,3 +27,11 @@ _load_package()
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    """The flame3d_amd package with its abi and surrogate modules loaded."""
+    import flame3d_amd.abi  # noqa: F401
+    import flame3d_amd.surrogate  # noqa: F401
+    return sys.modules["flame3d_amd"]

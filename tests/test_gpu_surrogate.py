@@ -1,0 +1,412 @@
+"""GPU tests of the learned chemistry surrogate (BF16 MFMA MLP).
+
+  * forward, gradient and Adam kernels against the FP64 MLP oracle
+    (oracle/mlp.py, pinned by finite differences), with the oracle emulating
+    the kernels' BF16 roundings (bars written in each test);
+  * training by the library's kernels reduces the loss of a teacher network;
+  * the fused surrogate chemistry step: post_predict, the element-conserving
+    projection and the FP64 energy bookkeeping against a host restatement
+    (1e-12); cells that fail a gate end bit-identical to the Newton step;
+    the backward-Euler residual gate agrees with the oracle's residual;
+  * the trained H2 surrogate on fresh reactors: species and temperature error
+    against the Newton step, the fallback fraction, and the Newton work it
+    removes (what the network gains over the integrator alone).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+SIZE = (16, 16, 8)
+
+
+def _abi():
+    import flame3d_amd.abi as abi
+    return abi
+
+
+def _mech():
+    import flame3d_amd.mech as mm
+    return mm.load("h2_lidryer")
+
+
+def _ctx(size=(8, 8, 8)):
+    abi = _abi()
+    ctx = abi.Context(*size, _mech(), "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    for k in range(9):
+        cfg.out_std[k] = 1.0
+    cfg.d_max = 1e30
+    cfg.elem_tol = 1e30
+    cfg.neg_tol = 1.0
+    cfg.boxcox_lambda = 0.1
+    cfg.dt = 1e-7
+    ctx.nn_init(cfg)
+    return ctx, cfg
+
+
+def _rand_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    layers = []
+    for ni, no in mlp.LAYERS:
+        W = np.zeros((no, ni))
+        fi = 11 if ni == mlp.NN_INP else ni
+        fo = 9 if no == mlp.NN_OUTP else no
+        W[:fo, :fi] = rng.normal(0, scale, (fo, fi))
+        b = np.zeros(no)
+        b[:fo] = rng.normal(0, 0.1, fo)
+        layers.append((W, b))
+    return mlp.pack(layers).astype(np.float32)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_matches_oracle():
+    print("\n[surrogate] forward B=256", flush=True)
+    ctx, _ = _ctx()
+    p = _rand_params(1)
+    ctx.nn_set_params(p)
+    assert np.array_equal(ctx.nn_get_params(), p)
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (256, 11)).astype(np.float32)
+    y = ctx.nn_forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    ye, _ = mlp.forward(p.astype(np.float64), X.astype(np.float64), emulate_bf16=True)
+    y64, _ = mlp.forward(p.astype(np.float64), X.astype(np.float64))
+    # same BF16 roundings, FP32 vs FP64 accumulation: rare one-ulp flips
+    assert _rel(y, ye) < 5e-3, _rel(y, ye)
+    assert np.max(np.abs(y - ye)) < 3e-2 * np.sqrt(np.mean(ye ** 2))
+    # against the exact FP64 network: the BF16 approximation itself
+    assert _rel(y, y64) < 3e-2, _rel(y, y64)
+    ctx.close()
+
+
+def test_gradient_matches_oracle():
+    print("\n[surrogate] gradient B=512", flush=True)
+    ctx, _ = _ctx()
+    p = _rand_params(3)
+    ctx.nn_set_params(p)
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (512, 11)).astype(np.float32)
+    T = rng.normal(0, 1, (512, 9)).astype(np.float32)
+    g = ctx.nn_grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()).cpu().numpy()
+    le, ge = mlp.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64),
+                           emulate_bf16=True)
+    l64, g64 = mlp.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert abs(g[-1] - le) / le < 1e-3, (g[-1], le)
+    for l, (ni, no) in enumerate(mlp.LAYERS):
+        w = slice(mlp.W_OFF[l], mlp.W_OFF[l] + ni * no)
+        b = slice(mlp.B_OFF[l], mlp.B_OFF[l] + no)
+        assert _rel(g[w], ge[w]) < 2e-2, (l, "W", _rel(g[w], ge[w]))
+        assert _rel(g[b], ge[b]) < 2e-2, (l, "b", _rel(g[b], ge[b]))
+        assert _rel(g[w], g64[w]) < 6e-2, (l, "W64", _rel(g[w], g64[w]))
+    # padded weights get exactly zero gradient
+    pad = [mlp.W_OFF[0] + o * 32 + k for o in range(64) for k in range(11, 32)]
+    pad += list(range(mlp.W_OFF[3] + 9 * 64, mlp.B_OFF[3])) + list(range(mlp.B_OFF[3] + 9, 12512))
+    assert np.all(g[pad] == 0)
+    ctx.close()
+
+
+def test_adam_steps():
+    print("\n[surrogate] adam", flush=True)
+    ctx, _ = _ctx()
+    p0 = _rand_params(5)
+    ctx.nn_set_params(p0)
+    rng = np.random.default_rng(6)
+    X = torch.from_numpy(rng.normal(0, 1, (256, 11)).astype(np.float32)).cuda()
+    T = torch.from_numpy(rng.normal(0, 1, (256, 9)).astype(np.float32)).cuda()
+    lr = 1e-3
+    g1 = ctx.nn_grad(X, T).cpu().numpy()[:-1].astype(np.float64)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    ctx.nn_train_step(X, T, lr, loss)
+    p1 = ctx.nn_get_params().astype(np.float64)
+    # first Adam step in closed form: -lr g / (|g| + eps)
+    exp1 = p0 - lr * g1 / (np.abs(g1) + 1e-8)
+    assert np.max(np.abs(p1 - exp1)) < 1e-6 * lr + 1e-6 * np.max(np.abs(p0))
+    # second step against the oracle's Adam with the GPU's own gradient
+    g2 = ctx.nn_grad(X, T).cpu().numpy()[:-1].astype(np.float64)
+    _, m1, m2 = mlp.adam_step(p0.astype(np.float64), 0 * g1, 0 * g1, g1, 1, lr)
+    exp2, _, _ = mlp.adam_step(p1, m1, m2, g2, 2, lr)
+    ctx.nn_train_step(X, T, lr, loss)
+    p2 = ctx.nn_get_params().astype(np.float64)
+    assert np.max(np.abs(p2 - exp2)) < 1e-5 * lr + 1e-6 * np.max(np.abs(p0))
+    pad = [mlp.W_OFF[0] + o * 32 + k for o in range(64) for k in range(11, 32)]
+    assert np.all(p2[pad] == 0)
+    ctx.close()
+
+
+def test_training_reduces_loss():
+    print("\n[surrogate] training on a teacher network", flush=True)
+    ctx, _ = _ctx()
+    rng = np.random.default_rng(7)
+    teacher = _rand_params(8, scale=0.25).astype(np.float64)
+    X = rng.normal(0, 1, (8192, 11))
+    Y, _ = mlp.forward(teacher, X)
+    Y = (Y - Y.mean(0)) / Y.std(0)
+    import flame3d_amd.surrogate as sur
+    ctx.nn_set_params(sur.init_params(9))
+    Xt = torch.from_numpy(X.astype(np.float32)).cuda()
+    Yt = torch.from_numpy(Y.astype(np.float32)).cuda()
+    l0 = sur.mse(ctx, Xt, Yt)
+    hist = sur.train(ctx, Xt, Yt, epochs=40, batch=1024, lr=3e-3, decay_every=20, log=None)
+    l1 = sur.mse(ctx, Xt, Yt)
+    print(f"  mse {l0:.3e} -> {l1:.3e}", flush=True)
+    # 320 Adam steps fit a random 3-hidden-layer teacher to ~10% of its variance
+    assert l1 < 0.25 * l0
+    assert hist[-1] < 0.25 * hist[0]
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# fused surrogate chemistry step
+# ---------------------------------------------------------------------------
+def _cells(seed=12345, burn_steps=0):
+    import flame3d_amd.cases as cases
+    m = _mech()
+    Q, r = cases.chem_cells(SIZE, m, seed=seed, dev="cuda")
+    U = cases.prim2c(Q.cpu()).cuda().contiguous()
+    if burn_steps:
+        abi = _abi()
+        c = abi.Context(*SIZE, m, "fp64")
+        for _ in range(burn_steps):
+            c.chem_newton(U, Q, r, 1e-6, 50, 1e-10, 1e-6)
+        c.close()
+    return m, Q, r, U
+
+
+def _projection(m, cfg):
+    comp = np.asarray(m.composition, np.float64)
+    E = comp / np.asarray(m.mw)[None, :]
+    S = np.diag(np.array(list(cfg.out_std), np.float64) ** 2)
+    G = E @ S @ E.T
+    return np.eye(9) - S @ E.T @ np.linalg.inv(G) @ E
+
+
+def _interior(t):
+    import flame3d_amd.surrogate as sur
+    return sur.interior(t).cpu().numpy()
+
+
+def test_nn_chem_all_fallback_equals_newton():
+    print(f"\n[surrogate] nn_chem all-fallback size={SIZE}", flush=True)
+    abi = _abi()
+    m, Q, r, U = _cells()
+    ctx = abi.Context(*SIZE, m, "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    for k in range(9):
+        cfg.out_std[k] = 1.0
+    cfg.d_max = -1.0  # every cell out of distribution
+    cfg.elem_tol, cfg.neg_tol, cfg.boxcox_lambda, cfg.dt = 1e30, 1.0, 0.1, 1e-7
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(_rand_params(10))
+    tag = torch.zeros((Q.shape[1], Q.shape[2], Q.shape[3]), dtype=torch.int64, device="cuda")
+    tag[4 + 3, 4 + 2, 4:4 + 5] = 1  # five tagged cells stay untouched
+    Qn, rn, Un = Q.clone(), r.clone(), U.clone()
+    ctx.chem_newton(Un, Qn, rn, 1e-7, 50, 1e-10, 1e-6, tag=tag)
+    st = torch.zeros((SIZE[2], SIZE[1], SIZE[0]), dtype=torch.int8, device="cuda")
+    Q0 = Q.clone()
+    nfb = ctx.nn_chem(U, Q, r, 1e-7, 50, 1e-10, 1e-6, status=st, tag=tag)
+    torch.cuda.synchronize()
+    ncell = SIZE[0] * SIZE[1] * SIZE[2]
+    assert nfb == ncell - 5
+    assert int((st == 2).sum()) == ncell - 5 and int((st == 0).sum()) == 5
+    assert torch.equal(Q, Qn) and torch.equal(r, rn) and torch.equal(U, Un)
+    assert torch.equal(Q[:, 7, 6, 4:9], Q0[:, 7, 6, 4:9])
+    with pytest.raises(abi.F3DError):
+        ctx.nn_chem(U, Q, r, 2e-7)  # dt other than the trained step
+    ctx.close()
+
+
+def test_nn_chem_post_predict_and_projection():
+    print(f"\n[surrogate] nn_chem post_predict size={SIZE}", flush=True)
+    abi = _abi()
+    from oracle import Oracle
+    m, Q, r, U = _cells(seed=21)
+    ctx = abi.Context(*SIZE, m, "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    mean = [-1.0e-4, -8.0e-4, 9.0e-4, 1.0e-5, 1.0e-5, 2.0e-5, 1.0e-6, 1.0e-7, 3.0e-5]
+    std = [1e-3, 2e-3, 2e-3, 1e-4, 1e-4, 3e-4, 1e-5, 1e-6, 1e-5]
+    for k in range(9):
+        cfg.out_mean[k], cfg.out_std[k] = mean[k], std[k]
+    cfg.d_max, cfg.elem_tol, cfg.neg_tol, cfg.res_tol = 1e30, 1e30, 1.0, 0.0
+    cfg.boxcox_lambda, cfg.dt = 0.1, 1e-7
+    cfg.n_elem = 3
+    for e in range(3):
+        for k in range(9):
+            cfg.elem[e][k] = float(m.composition[e][k])
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(np.zeros(12512, np.float32))  # prediction = out_mean exactly
+    q0, r0, u0 = _interior(Q), _interior(r), _interior(U)
+    st = torch.zeros((SIZE[2], SIZE[1], SIZE[0]), dtype=torch.int8, device="cuda")
+    nfb = ctx.nn_chem(U, Q, r, 1e-7, status=st)
+    torch.cuda.synchronize()
+    assert nfb == 0 and bool((st == 1).all())
+    q1, r1, u1 = _interior(Q), _interior(r), _interior(U)
+    Pm = _projection(m, cfg)
+    dY = Pm @ np.array(mean, np.float32).astype(np.float64)
+    assert abs(dY[8]) < 1e-18  # N2 is inert: its increment is projected out
+    comp = np.asarray(m.composition) / np.asarray(m.mw)[None, :]
+    assert np.max(np.abs(comp @ dY)) < 1e-12 * np.max(np.abs(comp @ np.abs(dY)))
+    orc = Oracle(*SIZE, m)
+    lo6 = np.asarray(m.coeffs_lo)[5] * m.Ru / np.asarray(m.mw)
+    worst = 0.0
+    for c in range(0, q0.shape[0], 7):
+        rho = q0[c, 0]
+        Y = np.maximum(r0[c] / rho, 0)
+        rn = np.maximum(Y + dY, 0) * rho
+        dei = -np.sum(lo6 * (rn - r0[c]))
+        rn = rn * (rho / rn.sum())
+        ein = q0[c, 6] + dei
+        T, _ = orc.get_T(ein, rn)
+        p = m.Ru * T * np.sum(rn / np.asarray(m.mw))
+        e_sp = np.max(np.abs(r1[c] - rn)) / rho
+        exp = np.array([T, p, ein, u0[c, 4] + dei])
+        got = np.array([q1[c, 5], q1[c, 4], q1[c, 6], u1[c, 4]])
+        worst = max(worst, e_sp, np.max(np.abs(got - exp) / np.abs(exp)))
+    assert worst < 1e-12, worst
+    ctx.close()
+
+
+def test_nn_chem_residual_gate():
+    print(f"\n[surrogate] nn_chem residual gate size={SIZE}", flush=True)
+    abi = _abi()
+    from oracle import Oracle
+    m, Q, r, U = _cells(seed=33)
+    ctx = abi.Context(*SIZE, m, "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    for k in range(9):
+        cfg.out_std[k] = 1.0
+    tol = 1e-8
+    cfg.d_max, cfg.elem_tol, cfg.neg_tol, cfg.res_tol = 1e30, 1e30, 1.0, tol
+    cfg.boxcox_lambda, cfg.dt = 0.1, 1e-7
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(np.zeros(12512, np.float32))  # identity prediction dY = 0
+    Qn, rn, Un = Q.clone(), r.clone(), U.clone()
+    ctx.chem_newton(Un, Qn, rn, 1e-7, 50, 1e-10, 1e-6)
+    q0, r0 = _interior(Q), _interior(r)
+    st = torch.zeros((SIZE[2], SIZE[1], SIZE[0]), dtype=torch.int8, device="cuda")
+    nfb = ctx.nn_chem(U, Q, r, 1e-7, 50, 1e-10, 1e-6, status=st)
+    torch.cuda.synchronize()
+    s = st.cpu().numpy().reshape(-1)
+    assert nfb == int((s >= 2).sum()) and 0 < nfb < s.size
+    assert set(np.unique(s).tolist()) <= {1, 5}  # only the residual gate is active
+    q1, r1, qn, rnn = _interior(Q), _interior(r), _interior(Qn), _interior(rn)
+    # rejected cells are exactly the Newton result
+    rej = s == 5
+    assert np.array_equal(q1[rej], qn[rej]) and np.array_equal(r1[rej], rnn[rej])
+    orc = Oracle(*SIZE, m)
+    mw = np.asarray(m.mw)
+    checked = 0
+    for c in range(0, s.size, 3):
+        rho = q0[c, 0]
+        rr = np.maximum(r0[c], 0)
+        rr = rr * (rho / rr.sum())
+        T, _ = orc.get_T(q0[c, 6], rr)
+        w = orc.production_rate(rr / mw * 1e-6, T, jac=False)
+        R = np.max(np.abs(rr - r0[c] - w * mw * 1e6 * 1e-7)) / rho
+        if abs(R / tol - 1) < 1e-6:
+            continue
+        assert (s[c] == 1) == (R <= tol), (c, R, s[c])
+        checked += 1
+    assert checked > s.size // 4
+    ctx.close()
+
+
+SURROGATE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                         "flame3d.jl_amd", "mechanisms", "h2_surrogate.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(SURROGATE), reason="trained surrogate not present")
+def test_trained_surrogate_accuracy_and_gain():
+    """Fresh reactors (seed unseen in training) at 64^3: surrogate step with
+    gates and Newton fallback vs the Newton step alone.  Bars: species
+    mass-fraction error <= 1e-3 and temperature error <= 1 K (Linf over all
+    cells); the network must take over at least half of the cells and of the
+    Newton iterations, and the step must run faster than Newton alone."""
+    import flame3d_amd.cases as cases
+    import flame3d_amd.surrogate as sur
+    abi = _abi()
+    shape = (64, 64, 64)
+    print(f"\n[surrogate] trained network on {shape} fresh reactors", flush=True)
+    m = _mech()
+    ctx = abi.Context(*shape, m, "fp64")
+    sur.attach(ctx, SURROGATE)
+    Q, r = cases.chem_cells(shape, m, seed=2024, dev="cuda")
+    U = torch.zeros((5,) + tuple(Q.shape[1:]), dtype=torch.float64, device="cuda")
+    Qn, rn, Un = Q.clone(), r.clone(), U.clone()
+    it = torch.zeros((shape[2], shape[1], shape[0]), dtype=torch.int32, device="cuda")
+    ctx.chem_newton(Un, Qn, rn, sur.DT, 50, 1e-10, 1e-6, iters=it)
+    st = torch.zeros_like(it, dtype=torch.int8)
+    nfb = ctx.nn_chem(U, Q, r, sur.DT, 50, 1e-10, 1e-6, status=st)
+    torch.cuda.synchronize()
+    Ye = (sur.mass_fractions(Q, r) - sur.mass_fractions(Qn, rn)).abs().max().item()
+    Te = (sur.interior(Q)[:, 5] - sur.interior(Qn)[:, 5]).abs().max().item()
+    its = it.reshape(-1).double()
+    avoided = float(its[st.reshape(-1) == 1].sum() / its.sum())
+    # timing: three repetitions each, from the same fresh state
+    Qf0, rf0 = cases.chem_cells(shape, m, seed=2024, dev="cuda")
+    times = {}
+    for name in ("newton", "surrogate"):
+        ts = []
+        for _ in range(3):
+            Qf, rf, Uf = Qf0.clone(), rf0.clone(), torch.zeros_like(U)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            if name == "newton":
+                ctx.chem_newton(Uf, Qf, rf, sur.DT, 50, 1e-10, 1e-6)
+            else:
+                ctx.nn_chem(Uf, Qf, rf, sur.DT, 50, 1e-10, 1e-6)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        times[name] = min(ts)
+    frac = nfb / st.numel()
+    print(f"  Y linf {Ye:.2e}  T linf {Te:.3f} K  fallback {frac:.3f}  newton iterations "
+          f"avoided {avoided:.3f}  newton {times['newton']:.2f} ms  surrogate "
+          f"{times['surrogate']:.2f} ms", flush=True)
+    assert Ye <= 1e-3
+    assert Te <= 1.0
+    assert frac <= 0.5
+    assert avoided >= 0.5
+    assert times["surrogate"] < times["newton"]
+    ctx.close()
+
+
+def test_train_on_integrator_data():
+    """End to end at small scale: integrator data -> statistics -> training
+    by the library's kernels -> surrogate step; the held-out loss must fall
+    well below the untrained network's."""
+    import flame3d_amd.surrogate as sur
+    abi = _abi()
+    m = _mech()
+    print("\n[surrogate] small end-to-end training", flush=True)
+    data = sur.generate(m, (16, 16, 8), steps=200, n_record=8, seed=5, log=None)
+    assert data.X.shape == (8 * 2048, 11) and data.Y1.shape == (8 * 2048, 9)
+    cfg = sur.make_config(m, data)
+    assert cfg.d_max > 0 and cfg.n_elem == 3
+    Xn, Yn = sur.normalize(cfg, data)
+    ctx = abi.Context(8, 8, 8, m, "fp64")
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(sur.init_params(1))
+    Xv, Yv = Xn[:1024].contiguous(), Yn[:1024].contiguous()
+    l0 = sur.mse(ctx, Xv, Yv)
+    sur.train(ctx, Xn[1024:].contiguous(), Yn[1024:].contiguous(), epochs=30, batch=1024,
+              lr=3e-3, decay_every=15, log=None)
+    l1 = sur.mse(ctx, Xv, Yv)
+    print(f"  held-out mse {l0:.3e} -> {l1:.3e}", flush=True)
+    assert l1 < 0.2 * l0
+    ctx.close()
