@@ -1,19 +1,24 @@
-"""Benchmark: cell-updates/s per RK stage of the per-stage RHS on one MI355X.
+"""Benchmark: cell-updates/s per RK stage of the per-stage RHS with chemistry,
+FP32 flux + learned chemistry vs the plain FP64 method, on one MI355X.
 
-Workload: the shock-laden mixing layer on a 256^3 curvilinear block
-(tanh-stretched y, factor 2.5), 5-species air, hybrid UP7 / WENO5-Z / NND
-with the pressure sensor, viscous + species diffusion, non-reacting; inputs
-generated in HBM (synthetic, seeded).  A step is one RK3 stage
-(solver.jl:298-326) over the whole block: mixture, sensor, species and flow
-convective sweeps, viscous fluxes, divergence, linComb, c2Prim, fills.
+Workload: the reacting H2/air shock-laden mixing layer on a 256^3 curvilinear
+block (tanh-stretched y, factor 2.5): 9-species H2/O2 mechanism, hybrid UP7 /
+WENO5-Z / NND with the pressure sensor, viscous + species diffusion, and
+Strang-split chemistry (solver.jl:240-378): per time step, chemistry over
+dt/2, the three RK3 stages, chemistry over dt/2 (dt = 2e-7 s at CFL 0.5).
+Inputs are generated in HBM (synthetic, seeded).
 
-`value` is the FP32-flux path (FP32 flux / viscous / divergence arithmetic,
-FP64 state, FP64 e->T Newton); the FP64 path is timed beside it, and the
-FP32 fields after the timed stages are compared with the FP64 ones
-(normwise relative L2 / Linf per field; the FP64 path itself matches the CPU
-oracle to 1e-15 in tests/test_gpu_parity.py).
+  * FP64 plain: FP64 stage arithmetic and the FP64 backward-Euler Newton
+    chemistry on every cell;
+  * FP32 + NN (`value`): FP32 flux / viscous / divergence arithmetic (FP64
+    state, FP64 e -> T Newton) and the BF16 MFMA MLP chemistry step with its
+    gates and the FP64 Newton fallback.
 
-    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--size 256]
+Both start from the same state; after the timed steps the FP32+NN fields are
+compared with the FP64-plain ones (the FP64 path matches the CPU oracle to
+1e-12 in tests/).  Cell-updates per RK stage = cells x 3 / time step.
+
+    python bench.py [--gpus 1] [--steps 6] [--warmup 2] [--size 256]
 """
 from __future__ import annotations
 
@@ -47,24 +52,26 @@ def log(*a):
 
 # ---------------------------------------------------------------------------
 # Algorithmic bytes per launch of each kernel: the compulsory reads and
-# writes of its inputs/outputs once each (state, metrics and J in double;
+# writes of its inputs/outputs once each (state in double; metrics and J in
+# M bytes: double on the FP64 path, the float copies on the FP32 path;
 # intermediates in the path's precision R bytes).  npts = padded points,
 # nf = faces of the sweep, nc = interior cells.
 # ---------------------------------------------------------------------------
 def kernel_bytes(name, R, Ns, npts, nf, nc):
     D = 8
+    M = R
     table = {
-        "flux_split": npts * (9 * D + 10 * R),            # Q(6) + 3 metrics -> Fp, Fm (5 each)
-        "species_split": npts * ((6 + Ns) * D + 2 * Ns * R),  # u,v,w + 3 metrics + rho_k -> 2Ns
-        "shock_sensor": npts * (D + R),                    # p -> phi
-        "weno5": nf * 16 * R,                              # Fp, Fm (10) + phi -> F (5)
+        "flux_split": npts * (6 * D + 3 * M + 10 * R),       # Q(6) + 3 metrics -> Fp, Fm (5 each)
+        "species_split": npts * ((3 + Ns) * D + 3 * M + 2 * Ns * R),  # u,v,w, rho_k, 3 metrics
+        "shock_sensor": npts * (D + R),                        # p -> phi
+        "weno5": nf * 16 * R,                                  # Fp, Fm (10) + phi -> F (5)
         "weno_species": nf * (2 * Ns + 1 + Ns) * R,
-        "viscous": nf * (14 * D + 9 * R),                  # u,v,w,T + 9 metrics + J; lam, mu, Fh(3) -> Fv(4)
-        "spec_viscous": nf * (12 * D + (3 * Ns + 3) * R),  # rho, T + 9 metrics + J; Yi, D -> Fd, Fh
+        "viscous": nf * (4 * D + 10 * M + 9 * R),              # u,v,w,T; 9 metrics + J; lam, mu, Fh(3) -> Fv(4)
+        "spec_viscous": nf * (2 * D + 10 * M + (3 * Ns + 3) * R),  # rho, T; metrics, J; Yi, D -> Fd, Fh
         "mixture": npts * ((3 + Ns) * D + (2 * Ns + 2) * R),
-        "div": nc * (27 * R + 11 * D),                     # 3x(5+4) faces, J, U RMW
+        "div": nc * (27 * R + 11 * D),                         # 3x(5+4) faces, J, U RMW
         "div_specs": nc * (6 * Ns * R + (1 + 2 * Ns) * D),
-        "c2prim": nc * ((10 + 2 * Ns + 2) * D),            # U, rho_k -> Q(7), rho_k
+        "c2prim": nc * ((10 + 2 * Ns + 2) * D),                # U, rho_k -> Q(7), rho_k
     }
     return table[name]
 
@@ -116,13 +123,17 @@ def traffic_for(kernel_symbol):
     return None if t is None else t["fetch_bytes"] + t["write_bytes"]
 
 
-# kernel symbol (as rocprofv3 prints it) of each timed kernel on the FP32 path
-KERNEL_SYMBOL = {
-    "flux_split": "void f3d::k_flux_split<float>", "species_split": "void f3d::k_species_split<float>",
-    "weno5": "void f3d::k_weno<float>", "weno_species": "void f3d::k_weno<float>",
-    "viscous": "void f3d::k_viscous<float>", "spec_viscous": "void f3d::k_spec_viscous<5, float>",
-    "shock_sensor": "void f3d::k_shock_sensor<float>", "mixture": "void f3d::k_mixture<5, float>",
-}
+def kernel_symbol(name, Ns):
+    """Kernel symbol (as rocprofv3 prints it) of each timed kernel on the FP32 path."""
+    return {
+        "flux_split": "void f3d::k_flux_split<float>",
+        "species_split": "void f3d::k_species_split<float>",
+        "weno5": "void f3d::k_weno<float>", "weno_species": "void f3d::k_weno<float>",
+        "viscous": "void f3d::k_viscous<float>",
+        "spec_viscous": f"void f3d::k_spec_viscous<{Ns}, float>",
+        "shock_sensor": "void f3d::k_shock_sensor<float>",
+        "mixture": f"void f3d::k_mixture<{Ns}, float>",
+    }[name]
 
 
 def time_kernels(ctx, torch, Q, rhoi, metrics, J, reps=10):
@@ -198,7 +209,24 @@ def calibrate_copy(torch, nbytes=1 << 30, reps=5):
                       "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
 
 
-def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps):
+def strang_step(ctx, st, dt, chem):
+    """One time step as solver.jl:240-378: chemistry over dt/2 and ghost
+    fill, the three RK3 stages (each ends with c2Prim and the fills), then
+    chemistry over dt/2 and the fill."""
+    U, Un, rhoi, rhon, Q = st
+    chem(dt / 2)
+    ctx.fill(Q, U, rhoi)
+    for k in (1, 2, 3):
+        ctx.rk3_stage(k, U, Un, rhoi, rhon, Q, dt)
+    chem(dt / 2)
+    ctx.fill(Q, U, rhoi)
+
+
+def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate=None):
+    """Time `steps` full time steps of the path.  prec fp64: FP64 stage +
+    FP64 backward-Euler Newton chemistry (the plain method); fp32 with a
+    surrogate: FP32 flux arithmetic + the BF16 MLP chemistry step with Newton
+    fallback.  Each chemistry call is also timed on its own (HIP events)."""
     N = case.grid.Nx, case.grid.Ny, case.grid.Nz
     ctx = abi.Context(*N, mech, prec)
     Q = case.Q.clone()
@@ -208,27 +236,49 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps):
     Un, rhon = ctx.zeros(5), ctx.zeros(mech.Ns)
     ctx.set_metrics(case.grid.metrics, case.grid.J)
     ctx.set_boundary(case.bc)
-    KRK = [1, 2, 3]
+    st = (U, Un, rhoi, rhon, Q)
+    chem_ms, fallback = [], []
+    if surrogate is not None:
+        surrogate.attach(ctx)
+
+        def chem(h):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fallback.append(ctx.nn_chem(U, Q, rhoi, h, 50, 1e-10, 1e-6))
+            e1.record()
+            chem_ms.append((e0, e1))
+    else:
+        def chem(h):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.chem_newton(U, Q, rhoi, h, 50, 1e-10, 1e-6)
+            e1.record()
+            chem_ms.append((e0, e1))
     for s in range(warmup):
-        ctx.rk3_stage(KRK[s % 3], U, Un, rhoi, rhon, Q, case.dt)
+        strang_step(ctx, st, case.dt, chem)
     torch.cuda.synchronize()
-    # restart from the initial state so both precisions advance identically
+    # restart from the initial state so both paths advance identically
     Q.copy_(case.Q)
     rhoi.copy_(case.rhoi)
     ctx.prim2c(U, Q)
+    chem_ms.clear()
+    fallback.clear()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
     for s in range(steps):
-        ctx.rk3_stage(KRK[s % 3], U, Un, rhoi, rhon, Q, case.dt)
+        strang_step(ctx, st, case.dt, chem)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
+    cms = [a.elapsed_time(b) for a, b in chem_ms]
     kt = time_kernels(ctx, torch, Q, rhoi, case.grid.metrics, case.grid.J, kernel_reps)
-    return ctx, dict(ms_total=ms, wall_s=wall, Q=Q, rhoi=rhoi, U=U, kernels=kt)
+    return ctx, dict(ms_total=ms, wall_s=wall, Q=Q, rhoi=rhoi, U=U, kernels=kt,
+                     chem_ms_mean=sum(cms) / len(cms), chem_calls=len(cms),
+                     fallback=fallback)
 
 
 def field_errors(torch, a, b, names):
@@ -245,16 +295,17 @@ def field_errors(torch, a, b, names):
     return out
 
 
-def cpu_baseline(mech, size, stages, min_seconds=10.0):
+def cpu_baseline(mech, size, steps, min_seconds=10.0):
     """The FP64 oracle (OpenMP over the host cores) on a bounded sample of the
-    same workload: a size^3 mixing-layer block, at least `stages` RK3 stages
-    and at least `min_seconds` of CPU work."""
+    same workload: a size^3 block of the reacting mixing layer, whole time
+    steps (Newton chemistry over dt/2, three RK3 stages, Newton over dt/2), at
+    least `steps` steps and at least `min_seconds` of CPU work."""
     sys.path.insert(0, ROOT)
     import numpy as np
     import flame3d_amd.cases as cases
     from oracle import Oracle
 
-    case = cases.mixing_layer((size, size, size), mech, dev="cpu")
+    case = cases.reacting_mixing_layer((size, size, size), mech, dev="cpu")
 
     def f(t):
         return np.array(t.numpy().T, order="F", copy=True)
@@ -264,28 +315,38 @@ def cpu_baseline(mech, size, stages, min_seconds=10.0):
     M, J = [f(m) for m in case.grid.metrics], f(case.grid.J)
     U = orc.prim2c(Q)
     Un, rn, ws = orc.zeros(5), orc.zeros(mech.Ns), orc.workspace()
-    orc.rk3_stage(1, U, Un, r, rn, Q, ws, M, J, case.dt, case.bc)  # warm-up
+
+    def step():
+        orc.chem_newton(U, Q, r, case.dt / 2)
+        orc.fill(case.bc, Q, U, r)
+        for k in (1, 2, 3):
+            orc.rk3_stage(k, U, Un, r, rn, Q, ws, M, J, case.dt, case.bc)
+        orc.chem_newton(U, Q, r, case.dt / 2)
+        orc.fill(case.bc, Q, U, r)
+
+    step()  # warm-up
     t0 = time.perf_counter()
     done = 0
-    while done < stages or time.perf_counter() - t0 < min_seconds:
-        orc.rk3_stage(done % 3 + 1, U, Un, r, rn, Q, ws, M, J, case.dt, case.bc)
+    while done < steps or time.perf_counter() - t0 < min_seconds:
+        step()
         done += 1
     dt = time.perf_counter() - t0
-    stages = done
-    return {"value": size ** 3 * stages / dt, "unit": "cell-updates/s", "cores": orc.num_threads(),
+    return {"value": 3 * size ** 3 * done / dt, "unit": "cell-updates/s", "cores": orc.num_threads(),
             "kind": "port",
-            "sample": f"FP64 oracle (OpenMP, {orc.num_threads()} threads), mixing layer "
-                      f"{size}^3 block, {stages} RK3 stages, {dt:.1f} s"}
+            "sample": f"FP64 oracle (OpenMP, {orc.num_threads()} threads), reacting H2/air mixing "
+                      f"layer {size}^3 block, {done} time steps (= {3 * done} RK3 stages with "
+                      f"Strang Newton chemistry), {dt:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6,
+                    help="timed time steps (each = 3 RK3 stages + 2 chemistry half-steps)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--cpu-size", type=int, default=96)
-    ap.add_argument("--cpu-stages", type=int, default=3)
+    ap.add_argument("--cpu-size", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -313,22 +374,33 @@ def main():
     import flame3d_amd.abi as abi
     import flame3d_amd.cases as cases
     import flame3d_amd.mech as mm
+    import flame3d_amd.surrogate as sur
 
-    mech = mm.load("air5")
+    mech = mm.load("h2_lidryer")
     N = (a.size, a.size, a.size)
-    log(f"[bench] building mixing-layer case {N} on the GPU")
-    case = cases.mixing_layer(N, mech, dev="cuda")
+    log(f"[bench] building the reacting mixing-layer case {N} on the GPU")
+    case = cases.reacting_mixing_layer(N, mech, dev="cuda")
     nc = N[0] * N[1] * N[2]
-    log(f"[bench] dt = {case.dt:.3e} s; timing FP64 path")
+    stages = 3 * a.steps
+    log(f"[bench] dt = {case.dt:.3e} s; FP64 plain path (FP64 stage + Newton chemistry)")
     c64, r64 = run_path("fp64", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps)
-    log(f"[bench] FP64 {r64['ms_total'] / a.steps:.3f} ms/stage; timing FP32 path")
-    c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps)
-    log(f"[bench] FP32 {r32['ms_total'] / a.steps:.3f} ms/stage")
+    ms64 = r64["ms_total"] / stages
+    log(f"[bench] FP64 plain {ms64:.3f} ms/stage (chemistry {r64['chem_ms_mean']:.2f} ms/call); "
+        "FP32 + surrogate path")
+    c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        surrogate=sur)
+    ms32 = r32["ms_total"] / stages
+    log(f"[bench] FP32+NN {ms32:.3f} ms/stage (chemistry {r32['chem_ms_mean']:.2f} ms/call, "
+        f"fallback {sum(r32['fallback']) / (nc * len(r32['fallback'])):.3f})")
     errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
-    errs.update(field_errors(torch, r32["rhoi"], r64["rhoi"], [f"rho_{s}" for s in mech.species]))
-
-    ms32 = r32["ms_total"] / a.steps
-    ms64 = r64["ms_total"] / a.steps
+    sl = (slice(None),) + (slice(4, -4),) * 3
+    Y32 = r32["rhoi"][sl] / r32["Q"][0][sl[1:]].unsqueeze(0)
+    Y64 = r64["rhoi"][sl] / r64["Q"][0][sl[1:]].unsqueeze(0)
+    dY = (Y32 - Y64).abs()
+    dT = (r32["Q"][5][sl[1:]] - r64["Q"][5][sl[1:]]).abs()
+    acc = {"Y_linf": float(dY.max()), "Y_l2": float(dY.pow(2).mean().sqrt()),
+           "T_linf_K": float(dT.max()), "T_l2_K": float(dT.pow(2).mean().sqrt()),
+           "per_species_Y_linf": {sp: float(dY[k].max()) for k, sp in enumerate(mech.species)}}
     value = nc / (ms32 * 1e-3)
 
     # per-kernel roofline (FP32 path); dominant = largest time per stage
@@ -347,27 +419,37 @@ def main():
     dk = ktab["fp32"][dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": dk["GBps"] / HBM_PEAK_GBS,
-                "traffic": traffic_for(KERNEL_SYMBOL[dom]),
+                "traffic": traffic_for(kernel_symbol(dom, Ns)),
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["ms"]}
 
     cpu = None
     if not a.no_cpu:
-        log(f"[bench] CPU baseline: oracle {a.cpu_size}^3 x {a.cpu_stages} stages")
-        cpu = cpu_baseline(mech, a.cpu_size, a.cpu_stages)
+        log(f"[bench] CPU baseline: oracle {a.cpu_size}^3 x >= {a.cpu_steps} steps")
+        cpu = cpu_baseline(mech, a.cpu_size, a.cpu_steps)
 
+    nfb = r32["fallback"]
     out = {
         "metric": "cell-updates/s per RK stage (FP32+NN vs FP64 plain) at stated error; % HBM peak",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"shock-laden mixing layer {N[0]}x{N[1]}x{N[2]}, tanh-stretched y "
-                               "(2.5), 5-species air non-reacting, hybrid UP7/WENO5-Z/NND, viscous",
-                   "cells": nc, "step": "one RK3 stage", "precision": "FP32 flux, FP64 state"},
+        "warmup": a.warmup, "ms_per_step": ms32 * 3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 flux + bf16 MLP + f64 chemistry/state",
+        "data": "synthetic",
+        "config": {"workload": f"reacting H2/air shock-laden mixing layer {N[0]}x{N[1]}x{N[2]}, "
+                               "tanh-stretched y (2.5), 9-species H2/O2 (Li-Dryer form), hybrid "
+                               "UP7/WENO5-Z/NND, viscous + diffusion, Strang chemistry",
+                   "cells": nc, "step": "one time step = chemistry dt/2 + 3 RK3 stages + "
+                   "chemistry dt/2; value = cells x 3 / step time",
+                   "dt_s": case.dt, "precision": "FP32 flux, FP64 state, BF16 MLP chemistry "
+                   "with FP64 Newton fallback"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fp64_plain": {"value": nc / (ms64 * 1e-3), "ms_per_step": ms64},
-        "speedup_fp32_vs_fp64": ms64 / ms32,
-        "error_fp32_vs_fp64": errs,
+        "fp64_plain": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
+                       "chem_ms_per_call": r64["chem_ms_mean"]},
+        "fp32_nn": {"ms_per_stage": ms32, "chem_ms_per_call": r32["chem_ms_mean"],
+                    "fallback_fraction": sum(nfb) / (nc * len(nfb))},
+        "speedup_vs_fp64_plain": ms64 / ms32,
+        "error_vs_fp64_plain": acc,
+        "error_fields_vs_fp64_plain": errs,
         "kernels": ktab,
     }
     print(json.dumps(out))

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libflame3d.so")
 
 F3D_FP64, F3D_FP32 = 0, 1
+CHEM_NONE, CHEM_STIFF, CHEM_EXPLICIT, CHEM_NEWTON, CHEM_SURROGATE = 0, 1, 2, 3, 4
 BC_KEEP, BC_PERIODIC, BC_COPY, BC_INFLOW = 0, 1, 2, 3
 NG = 4
 
@@ -61,7 +62,7 @@ class NNConfig(C.Structure):
                 ("out_mean", C.c_float * 9), ("out_std", C.c_float * 9),
                 ("mahal_mean", C.c_float * 11), ("mahal_inv", C.c_float * 121),
                 ("d_max", C.c_float), ("elem_tol", C.c_float), ("neg_tol", C.c_float),
-                ("res_tol", C.c_float),
+                ("res_tol", C.c_float), ("res_T_tol", C.c_float), ("dT_max", C.c_float),
                 ("boxcox_lambda", C.c_float), ("label_mode", C.c_int32),
                 ("label_scale", C.c_float * 9),
                 ("n_elem", C.c_int32),
@@ -104,6 +105,9 @@ _PROTOS = {
     "f3d_set_metrics": [_vp, _vp, _vp],
     "f3d_rk3_stage": [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp],
     "f3d_workspace": [_vp, C.c_char_p],
+    "f3d_time_step": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                      C.c_double, C.c_double, C.POINTER(C.c_int64), _vp, _vp],
+    "f3d_nan_check": [_vp, _vp, C.POINTER(C.c_int32), _vp],
     "f3d_last_error": [],
 }
 
@@ -437,6 +441,26 @@ class Context:
                                    self._pad(rhoi, Ns, "rhoi"), self._pad(rhon, Ns, "rhon"),
                                    self._pad(Q, 7, "Q"), dt, self._tag(tag), self.stream()),
             "f3d_rk3_stage")
+
+    def time_step(self, U, Un, rhoi, rhon, Q, dt, chem=CHEM_NONE, sub_step=1, maxiter=50,
+                  rtol=1e-10, ttol=1e-6, tag=None):
+        """One Strang time step (solver.jl:240-378); returns the surrogate's
+        fallback count (0 for the other chemistry forms)."""
+        if self._metrics_keep is None:
+            raise F3DError("set_metrics first")
+        Ns = self.Ns
+        nfb = C.c_int64(0)
+        _rc(self.lib.f3d_time_step(self.h, self._pad(U, 5, "U"), self._pad(Un, 5, "Un"),
+                                   self._pad(rhoi, Ns, "rhoi"), self._pad(rhon, Ns, "rhon"),
+                                   self._pad(Q, 7, "Q"), dt, chem, sub_step, maxiter, rtol, ttol,
+                                   C.byref(nfb), self._tag(tag), self.stream()), "f3d_time_step")
+        return int(nfb.value)
+
+    def nan_check(self, U) -> bool:
+        flag = C.c_int32(0)
+        _rc(self.lib.f3d_nan_check(self.h, self._pad(U, 5, "U"), C.byref(flag), self.stream()),
+            "f3d_nan_check")
+        return bool(flag.value)
 
     def workspace(self, name: str):
         """torch view of a context-owned intermediate after rk3_stage."""

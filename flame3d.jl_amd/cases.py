@@ -311,6 +311,63 @@ def mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.5,
     return Case("mixing_layer", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
 
 
+def reacting_mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.5,
+                          dt=2e-7, cfl=0.5, P_atm=3.0, T_air=1500.0, T_fuel=1000.0) -> Case:
+    """Reacting H2/air shock-laden mixing layer (the 9-species H2/O2 mechanism
+    with N2): a Ma 1.5 hot-air upper stream (T 1500 K) over a Ma 0.3 hydrogen
+    lower stream (T 1000 K) at 3 atm, mixture fraction Z = 1 - th(y) with the
+    tanh profile of thickness L/40 (Y = Z Y_H2 + (1 - Z) Y_air, T linear in Z,
+    the states of the H2/O2 reactor distribution), a normal shock at x = L/3 on
+    the supersonic stream (Rankine-Hugoniot, gamma 1.4; 7.4 atm, 1980 K behind
+    it), 1% seeded white-noise velocity.  The box edge L is chosen so that the
+    step `dt` is at CFL `cfl`, so that Strang half-steps of chemistry are dt/2.
+    Boundaries as mixing_layer."""
+    from . import abi
+    Nx, Ny, Nz = N
+
+    def build(L):
+        grid = stretched_y_grid(Nx, Ny, Nz, L, L, L, beta=beta, dev=dev)
+        gam = 1.4
+        W_air = 1.0 / (0.233 / mech.mw[mech.index("O2")] + 0.767 / mech.mw[mech.index("N2")])
+        W_h2 = float(mech.mw[mech.index("H2")])
+        c_up = math.sqrt(gam * mech.Ru / W_air * T_air)
+        c_lo = math.sqrt(gam * mech.Ru / W_h2 * T_fuel)
+        U_up, U_lo = 1.5 * c_up, 0.3 * c_lo
+        delta = L / 40
+        th = 0.5 * (1 + torch.tanh(grid.y / delta))  # 1 air (upper), 0 fuel (lower)
+        Z = 1.0 - th
+        T = T_fuel * Z + T_air * th
+        u = U_lo + (U_up - U_lo) * th
+        P = torch.full_like(T, 101325.0 * P_atm)
+        c = c_up * torch.sqrt(T / T_air)
+        M1 = u / c
+        post = (grid.x > L / 3) & (M1 > 1.0) & (th > 0.5)
+        M1s = torch.clamp(M1, min=1.0)
+        p_ratio = 1 + 2 * gam / (gam + 1) * (M1s ** 2 - 1)
+        r_ratio = (gam + 1) * M1s ** 2 / ((gam - 1) * M1s ** 2 + 2)
+        P = torch.where(post, P * p_ratio, P)
+        T = torch.where(post, T * p_ratio / r_ratio, T)
+        u = torch.where(post, u / r_ratio, u)
+        g = torch.Generator(device=dev).manual_seed(seed)
+        amp = 0.01 * U_up
+        u = u + amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+        v = amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+        w = amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+        Y = torch.zeros((mech.Ns,) + tuple(T.shape), dtype=torch.float64, device=dev)
+        Y[mech.index("H2")] = Z
+        Y[mech.index("O2")] = 0.233 * th
+        Y[mech.index("N2")] = 0.767 * th
+        Q, rhoi = state_from_TPY(mech, T, P, Y, u, v, w)
+        return grid, Q, rhoi
+
+    L0 = 0.05
+    grid, Q, rhoi = build(L0)
+    L = L0 * dt / cfl_dt(grid, Q, cfl)
+    grid, Q, rhoi = build(L)
+    bc = (abi.BC_KEEP, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY, abi.BC_PERIODIC, abi.BC_PERIODIC)
+    return Case("reacting_mixing_layer", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
+
+
 def taylor_green(N=64, mech=None, Ma=0.1, Re=1600.0, T0=300.0, cfl=0.5, dev="cpu") -> Case:
     """Taylor-Green vortex on the periodic [0, 2 pi L)^3 box with V0 = Ma c0,
     rho0 from p0 = rho0 R T0, mu from Re = rho0 V0 L / mu (L = 1 m)."""

@@ -244,9 +244,22 @@ int f3d_destroy(f3d_ctx* c) {
   if (!c) return F3D_OK;
   nn_free(c);
   if (c->ws_base) (void)hipFree(c->ws_base);
+  if (c->mf_base) (void)hipFree(c->mf_base);
+  if (c->nan_flag) (void)hipFree(c->nan_flag);
   if (c->dparams) (void)hipFree(c->dparams);
   delete c;
   return F3D_OK;
+}
+
+// On the FP32 path, metric arrays that are the ones registered by
+// f3d_set_metrics are read from the context's float copies (same values as
+// the kernels' own float conversion; half the bytes).  Returns the copy of
+// registered metric `m`, or nullptr.
+static const float* registered_float(const f3d_ctx* c, const double* m) {
+  if (c->precision != F3D_FP32 || !c->mf_base || !m) return nullptr;
+  for (int q = 0; q < 9; ++q)
+    if (c->metrics[q] == m) return c->mf[q];
+  return (c->J == m) ? c->mf[9] : nullptr;
 }
 
 int f3d_real_size(const f3d_ctx* c) { return c ? (int)real_size(c) : F3D_ERR_ARG; }
@@ -260,6 +273,9 @@ int f3d_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double
   CHECK_PTR(Ax, "Ax");
   CHECK_PTR(Ay, "Ay");
   CHECK_PTR(Az, "Az");
+  const float *fx = registered_float(c, Ax), *fy = registered_float(c, Ay),
+              *fz = registered_float(c, Az);
+  if (fx && fy && fz) return launch_flux_split(c, Q, Fp, Fm, fx, fy, fz, tag, s, true);
   return launch_flux_split(c, Q, Fp, Fm, Ax, Ay, Az, tag, s);
 }
 
@@ -274,6 +290,9 @@ int f3d_species_split(f3d_ctx* c, const double* rhoi, const double* Q, void* Fp,
   CHECK_PTR(Ax, "Ax");
   CHECK_PTR(Ay, "Ay");
   CHECK_PTR(Az, "Az");
+  const float *fx = registered_float(c, Ax), *fy = registered_float(c, Ay),
+              *fz = registered_float(c, Az);
+  if (fx && fy && fz) return launch_species_split(c, rhoi, Q, Fp, Fm, fx, fy, fz, tag, s, true);
   return launch_species_split(c, rhoi, Q, Fp, Fm, Ax, Ay, Az, tag, s);
 }
 
@@ -314,7 +333,17 @@ int f3d_viscous_flux(f3d_ctx* c, int32_t dir, void* Fv, const double* Q,
     set_error("f3d_viscous_flux: dir must be 0..2");
     return F3D_ERR_ARG;
   }
-  return launch_viscous(c, dir, Fv, Q, metrics, J, lambda, mu, Fh, tag, s);
+  const void* Mv[9];
+  const void* Mf[9];
+  bool all = true;
+  for (int q = 0; q < 9; ++q) {
+    Mv[q] = metrics[q];
+    Mf[q] = registered_float(c, metrics[q]);
+    all = all && Mf[q];
+  }
+  const float* Jf = registered_float(c, J);
+  if (all && Jf) return launch_viscous(c, dir, Fv, Q, Mf, Jf, lambda, mu, Fh, tag, s, true);
+  return launch_viscous(c, dir, Fv, Q, Mv, J, lambda, mu, Fh, tag, s);
 }
 
 int f3d_spec_viscous_flux(f3d_ctx* c, int32_t dir, void* Fd, const double* Q, const void* Yi,
@@ -333,7 +362,17 @@ int f3d_spec_viscous_flux(f3d_ctx* c, int32_t dir, void* Fd, const double* Q, co
     set_error("f3d_spec_viscous_flux: dir must be 0..2");
     return F3D_ERR_ARG;
   }
-  return launch_spec_viscous(c, dir, Fd, Q, Yi, metrics, J, D, Fh, tag, s);
+  const void* Mv[9];
+  const void* Mf[9];
+  bool all = true;
+  for (int q = 0; q < 9; ++q) {
+    Mv[q] = metrics[q];
+    Mf[q] = registered_float(c, metrics[q]);
+    all = all && Mf[q];
+  }
+  const float* Jf = registered_float(c, J);
+  if (all && Jf) return launch_spec_viscous(c, dir, Fd, Q, Yi, Mf, Jf, D, Fh, tag, s, true);
+  return launch_spec_viscous(c, dir, Fd, Q, Yi, Mv, J, D, Fh, tag, s);
 }
 
 int f3d_div(f3d_ctx* c, double* U, const void* Fx, const void* Fy, const void* Fz,
@@ -493,6 +532,12 @@ int f3d_chem_newton(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, i
   return launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, iters, tag, s);
 }
 
+#define RET(x)              \
+  do {                      \
+    int _rc = (x);          \
+    if (_rc) return _rc;    \
+  } while (0)
+
 int f3d_set_metrics(f3d_ctx* c, const double* const* metrics, const double* J) {
   CHECK_CTX(c);
   CHECK_PTR(metrics, "metrics");
@@ -500,6 +545,20 @@ int f3d_set_metrics(f3d_ctx* c, const double* const* metrics, const double* J) {
   CHECK_PTR(J, "J");
   for (int q = 0; q < 9; ++q) c->metrics[q] = metrics[q];
   c->J = J;
+  if (c->precision == F3D_FP32) {
+    const long NP = c->g.NP;
+    if (!c->mf_base) {
+      if (hipMalloc(&c->mf_base, sizeof(float) * NP * 10) != hipSuccess) {
+        c->mf_base = nullptr;
+        set_error("f3d_set_metrics: allocation of the float metric copies failed");
+        return F3D_ERR_ALLOC;
+      }
+      for (int q = 0; q < 10; ++q) c->mf[q] = (float*)c->mf_base + (size_t)q * NP;
+    }
+    for (int q = 0; q < 9; ++q) RET(launch_to_float(c, c->mf[q], metrics[q], NP, nullptr));
+    RET(launch_to_float(c, c->mf[9], J, NP, nullptr));
+    RET(hip_check(hipDeviceSynchronize(), "f3d_set_metrics"));
+  }
   return F3D_OK;
 }
 
@@ -509,12 +568,6 @@ void* f3d_workspace(f3d_ctx* c, const char* name) {
     if (std::strcmp(name, kWsNames[w]) == 0) return c->ws[w];
   return nullptr;
 }
-
-#define RET(x)              \
-  do {                      \
-    int _rc = (x);          \
-    if (_rc) return _rc;    \
-  } while (0)
 
 int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, double* rhon,
                   double* Q, double dt, const int64_t* tag, hipStream_t s) {
@@ -535,7 +588,11 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   RET(ensure_workspace(c));
   const Geom& g = c->g;
   void** w = c->ws;
-  const double* const* M = c->metrics;
+  // metrics: the caller's FP64 arrays, or the float copies on the FP32 path
+  const bool mf = (c->precision == F3D_FP32);
+  const void* M[9];
+  for (int q = 0; q < 9; ++q) M[q] = mf ? (const void*)c->mf[q] : (const void*)c->metrics[q];
+  const void* Jm = mf ? (const void*)c->mf[9] : (const void*)c->J;
   const double* J = c->J;
   if (KRK == 1) {
     RET(hip_check(hipMemcpyAsync(Un, U, sizeof(double) * (size_t)g.NP * 5,
@@ -548,20 +605,23 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   // specAdvance, solver.jl:40-56
   for (int d = 0; d < 3; ++d) {
     RET(launch_species_split(c, rhoi, Q, w[W_Fpi], w[W_Fmi], M[3 * d], M[3 * d + 1],
-                             M[3 * d + 2], tag, s));
+                             M[3 * d + 2], tag, s, mf));
     RET(launch_weno(c, d, w[W_Fxi + d], w[W_phi], w[W_Fpi], w[W_Fmi], g.Ns, tag, s));
   }
   for (int d = 0; d < 3; ++d)
-    RET(launch_spec_viscous(c, d, w[W_Fdx + d], Q, w[W_Yi], M, J, w[W_D], w[W_Fhx + d], tag, s));
+    RET(launch_spec_viscous(c, d, w[W_Fdx + d], Q, w[W_Yi], M, Jm, w[W_D], w[W_Fhx + d], tag, s,
+                            mf));
   RET(launch_div_specs(c, rhoi, w[W_Fxi], w[W_Fyi], w[W_Fzi], w[W_Fdx], w[W_Fdy], w[W_Fdz], dt,
                        J, tag, s));
   // flowAdvance, solver.jl:22-38
   for (int d = 0; d < 3; ++d) {
-    RET(launch_flux_split(c, Q, w[W_Fp], w[W_Fm], M[3 * d], M[3 * d + 1], M[3 * d + 2], tag, s));
+    RET(launch_flux_split(c, Q, w[W_Fp], w[W_Fm], M[3 * d], M[3 * d + 1], M[3 * d + 2], tag, s,
+                          mf));
     RET(launch_weno(c, d, w[W_Fx + d], w[W_phi], w[W_Fp], w[W_Fm], 5, tag, s));
   }
   for (int d = 0; d < 3; ++d)
-    RET(launch_viscous(c, d, w[W_Fvx + d], Q, M, J, w[W_lam], w[W_mu], w[W_Fhx + d], tag, s));
+    RET(launch_viscous(c, d, w[W_Fvx + d], Q, M, Jm, w[W_lam], w[W_mu], w[W_Fhx + d], tag, s,
+                       mf));
   RET(launch_div(c, U, w[W_Fx], w[W_Fy], w[W_Fz], w[W_Fvx], w[W_Fvy], w[W_Fvz], dt, J, tag, s));
   if (KRK == 2) {
     RET(launch_lincomb(c, U, Un, 5, 0.25, 0.75, tag, s));
@@ -573,6 +633,73 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   RET(launch_c2prim(c, U, Q, rhoi, tag, s));
   RET(launch_fill(c, Q, U, rhoi, s));
   return F3D_OK;
+}
+
+// One time step, solver.jl:240-378: chemistry over dt/2 (sub_step substeps
+// for the stiff / explicit forms, run.jl:23) and the fills, the three RK3
+// stages, chemistry over dt/2 and the fills.
+int f3d_time_step(f3d_ctx* c, double* U, double* Un, double* rhoi, double* rhon, double* Q,
+                  double dt, int32_t chem, int32_t sub_step, int32_t maxiter, double rtol,
+                  double ttol, int64_t* n_fallback, const int64_t* tag, hipStream_t s) {
+  CHECK_CTX(c);
+  if (chem < F3D_CHEM_NONE || chem > F3D_CHEM_SURROGATE || sub_step < 1) {
+    set_error("f3d_time_step: chem must be one of F3D_CHEM_* and sub_step >= 1");
+    return F3D_ERR_ARG;
+  }
+  auto half = [&]() -> int {
+    const double h = 0.5 * dt;
+    switch (chem) {
+      case F3D_CHEM_NONE: return F3D_OK;
+      case F3D_CHEM_STIFF:
+        for (int k = 0; k < sub_step; ++k)
+          RET(f3d_chem_stiff(c, U, Q, rhoi, h / sub_step, tag, s));
+        break;
+      case F3D_CHEM_EXPLICIT:
+        for (int k = 0; k < sub_step; ++k)
+          RET(f3d_chem_explicit(c, U, Q, rhoi, h / sub_step, tag, s));
+        break;
+      case F3D_CHEM_NEWTON:
+        RET(f3d_chem_newton(c, U, Q, rhoi, h, maxiter, rtol, ttol, nullptr, tag, s));
+        break;
+      default: {
+        int64_t nfb = 0;
+        RET(f3d_nn_chem(c, U, Q, rhoi, h, maxiter, rtol, ttol, nullptr, &nfb, tag, s));
+        if (n_fallback) *n_fallback += nfb;
+      }
+    }
+    return f3d_fill(c, Q, U, rhoi, s);
+  };
+  RET(half());
+  for (int k = 1; k <= 3; ++k) RET(f3d_rk3_stage(c, k, U, Un, rhoi, rhon, Q, dt, tag, s));
+  return half();
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_nan_check(const double* __restrict__ U, long n, int32_t* __restrict__ flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && U[i] != U[i]) flag[0] = 1;  // racing stores of the same value
+}
+
+int f3d_nan_check(f3d_ctx* c, const double* U, int32_t* flag, hipStream_t s) {
+  CHECK_CTX(c);
+  CHECK_PTR(U, "U");
+  CHECK_PTR(flag, "flag");
+  if (!c->nan_flag && hipMalloc((void**)&c->nan_flag, sizeof(int32_t)) != hipSuccess) {
+    c->nan_flag = nullptr;
+    set_error("f3d_nan_check: allocation failed");
+    return F3D_ERR_ALLOC;
+  }
+  int32_t* d = c->nan_flag;
+  RET(hip_check(hipMemsetAsync(d, 0, sizeof(int32_t), s), "f3d_nan_check"));
+  const long n = (long)c->g.NP * 5;
+  hipLaunchKernelGGL(k_nan_check, dim3(nblocks(n)), dim3(kBlock), 0, s, U, n, d);
+  int rc = hip_check(hipGetLastError(), "f3d_nan_check");
+  int32_t h = 0;
+  if (!rc) rc = hip_check(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, s),
+                          "f3d_nan_check");
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "f3d_nan_check");
+  *flag = h;
+  return rc;
 }
 
 }  // extern "C"

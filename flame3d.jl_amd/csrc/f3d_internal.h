@@ -58,6 +58,11 @@ struct f3d_ctx {
   // metrics registered by f3d_set_metrics
   const double* metrics[9] = {nullptr};
   const double* J = nullptr;
+  // F3D_FP32: float copies of the 9 metrics and J (same values as the
+  // float conversions the kernels would make; half the bytes)
+  float* mf[10] = {nullptr};
+  void* mf_base = nullptr;
+  int32_t* nan_flag = nullptr;  // f3d_nan_check's device flag
   // context-owned workspace for f3d_rk3_stage
   void* ws_base = nullptr;
   size_t ws_bytes = 0;
@@ -89,15 +94,16 @@ constexpr int kBlock = 256;
 
 inline unsigned nblocks(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
-// Bijective remap of blockIdx.x so that consecutive logical blocks run on the
-// same XCD (workgroups are dispatched round-robin over the 8 XCDs, each with
-// its own L2): neighbouring tiles then share an L2.
-__device__ inline unsigned xcd_block() {
-  const unsigned nb = gridDim.x, b = blockIdx.x;
+// Bijective remap of hardware block b of nb so that consecutive logical
+// blocks run on the same XCD (workgroups are dealt round-robin over the 8
+// XCDs, each with its own L2): neighbouring tiles then share an L2.
+__device__ inline unsigned xcd_remap(unsigned b, unsigned nb) {
   const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
   const unsigned base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
   return base + k;
 }
+
+__device__ inline unsigned xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
 __device__ inline long gid() { return (long)xcd_block() * blockDim.x + threadIdx.x; }
 
@@ -275,20 +281,24 @@ __device__ inline void chem_epilogue(const DevParams* __restrict__ P, double* __
 }
 
 // ---- internal launchers (defined in the .hip files) ------------------------
-int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
-                      const double* Ay, const double* Az, const int64_t* tag, hipStream_t s);
+// Metric arguments: `M` points at 9 metric arrays (and `J` at the Jacobian)
+// of type double, or of type float when `mf` is set (the FP32 stage's copies).
+int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const void* Ax,
+                      const void* Ay, const void* Az, const int64_t* tag, hipStream_t s,
+                      bool mf = false);
 int launch_species_split(f3d_ctx* c, const double* rhoi, const double* Q, void* Fp, void* Fm,
-                         const double* Ax, const double* Ay, const double* Az,
-                         const int64_t* tag, hipStream_t s);
+                         const void* Ax, const void* Ay, const void* Az,
+                         const int64_t* tag, hipStream_t s, bool mf = false);
 int launch_shock_sensor(f3d_ctx* c, void* phi, const double* Q, hipStream_t s);
 int launch_weno(f3d_ctx* c, int dir, void* F, const void* phi, const void* Fp, const void* Fm,
                 int NV, const int64_t* tag, hipStream_t s);
-int launch_viscous(f3d_ctx* c, int dir, void* Fv, const double* Q, const double* const* M,
-                   const double* J, const void* lam, const void* mu, const void* Fh,
-                   const int64_t* tag, hipStream_t s);
+int launch_viscous(f3d_ctx* c, int dir, void* Fv, const double* Q, const void* const* M,
+                   const void* J, const void* lam, const void* mu, const void* Fh,
+                   const int64_t* tag, hipStream_t s, bool mf = false);
 int launch_spec_viscous(f3d_ctx* c, int dir, void* Fd, const double* Q, const void* Yi,
-                        const double* const* M, const double* J, const void* D, void* Fh,
-                        const int64_t* tag, hipStream_t s);
+                        const void* const* M, const void* J, const void* D, void* Fh,
+                        const int64_t* tag, hipStream_t s, bool mf = false);
+int launch_to_float(f3d_ctx* c, float* dst, const double* src, long n, hipStream_t s);
 int launch_div(f3d_ctx* c, double* U, const void* Fx, const void* Fy, const void* Fz,
                const void* Fvx, const void* Fvy, const void* Fvz, double dt, const double* J,
                const int64_t* tag, hipStream_t s);

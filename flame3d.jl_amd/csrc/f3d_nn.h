@@ -56,6 +56,8 @@ struct NNDev {
   float elem_tol;                  // element-balance gate (relative)
   float neg_tol;                   // most negative admissible Y_k + dY_k
   float res_tol;                   // backward-Euler residual gate (<= 0: off)
+  float res_T_tol;                 // temperature-weighted residual gate [K] (<= 0: off)
+  float dT_max;                    // predicted chemical temperature change gate [K] (<= 0: off)
   float boxcox_lambda;             // Y -> (Y^lambda - 1)/lambda input transform
   int label_mode;                  // 0: dY; 1: d BoxCox(Y); 2: asinh(dY / label_scale)
   float label_scale[NN_OUT];

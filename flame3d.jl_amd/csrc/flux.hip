@@ -6,18 +6,19 @@
 // consecutive i of one variable plane (coalesced), with XCD-aware block
 // ordering so y/z stencil neighbours are reused from the same L2.
 // Template parameter R is the arithmetic/storage type of the intermediates
-// (double for the FP64 path, float for the FP32 path); the state Q, rho_k
-// and the metrics are read as double.
+// (double for the FP64 path, float for the FP32 path); the state Q and rho_k
+// are read as double, the metrics as MT (double, or the FP32 stage's float
+// copies).
 #include "f3d_internal.h"
 
 namespace f3d {
 
 // fluxSplit, split.jl:17-89
-template <typename R>
+template <typename R, typename MT>
 __global__ void __launch_bounds__(kBlock)
     k_flux_split(const double* __restrict__ Q, R* __restrict__ Fp, R* __restrict__ Fm,
-                 const double* __restrict__ Ax, const double* __restrict__ Ay,
-                 const double* __restrict__ Az, const int64_t* __restrict__ tag,
+                 const MT* __restrict__ Ax, const MT* __restrict__ Ay,
+                 const MT* __restrict__ Az, const int64_t* __restrict__ tag,
                  const DevParams* __restrict__ P, int NP) {
   const long p = gid();
   if (p >= NP) return;
@@ -62,11 +63,11 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // split, split.jl:101-130
-template <typename R>
+template <typename R, typename MT>
 __global__ void __launch_bounds__(kBlock)
     k_species_split(const double* __restrict__ rhoi, const double* __restrict__ Q,
-                    R* __restrict__ Fp, R* __restrict__ Fm, const double* __restrict__ Ax,
-                    const double* __restrict__ Ay, const double* __restrict__ Az,
+                    R* __restrict__ Fp, R* __restrict__ Fm, const MT* __restrict__ Ax,
+                    const MT* __restrict__ Ay, const MT* __restrict__ Az,
                     const int64_t* __restrict__ tag, int NP, int Ns) {
   const long p = gid();
   if (p >= NP) return;
@@ -139,22 +140,30 @@ __device__ inline R minmod(R a, R b) {
   return (a * b > R(0)) ? ((fabs(a) > fabs(b)) ? b : a) : R(0);
 }
 
-// WENO_x / WENO_y / WENO_z face value, schemes.jl:86-437.  fp/fm point at the
-// variable plane, ic is the face's right cell, s the sweep stride.
+// Branch of the hybrid reconstruction at a face (schemes.jl:96-100):
+// 0 upwind7, 1 WENO5-Z, 2 NND.
 template <typename R>
-__device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm, long ic, long s,
-                              R phif, const DevParams* __restrict__ P) {
+__device__ inline int weno_branch(R phif, const DevParams* __restrict__ P) {
+  return (phif < PT<R>::phi1(P)) ? 0 : (phif < PT<R>::phi2(P)) ? 1 : 2;
+}
+
+// WENO_x / WENO_y / WENO_z face value, schemes.jl:86-437, on the eight
+// values v[k] = f[ic + (k - 4) s] of Fp (p) and Fm (m) around the face i-1/2
+// (ic its right cell, s the sweep stride).
+template <typename R>
+__device__ inline R weno_face_v(const R (&p)[8], const R (&m)[8], int br,
+                                const DevParams* __restrict__ P) {
   const R* c = PT<R>::up7(P);
-  if (phif < PT<R>::phi1(P)) {
-    const R fpv = c[0] * fp[ic - 4 * s] + c[1] * fp[ic - 3 * s] + c[2] * fp[ic - 2 * s] +
-                  c[3] * fp[ic - s] + c[4] * fp[ic] + c[5] * fp[ic + s] + c[6] * fp[ic + 2 * s];
-    const R fmv = c[6] * fm[ic - 3 * s] + c[5] * fm[ic - 2 * s] + c[4] * fm[ic - s] +
-                  c[3] * fm[ic] + c[2] * fm[ic + s] + c[1] * fm[ic + 2 * s] + c[0] * fm[ic + 3 * s];
+  if (br == 0) {
+    const R fpv = c[0] * p[0] + c[1] * p[1] + c[2] * p[2] + c[3] * p[3] + c[4] * p[4] +
+                  c[5] * p[5] + c[6] * p[6];
+    const R fmv = c[6] * m[1] + c[5] * m[2] + c[4] * m[3] + c[3] * m[4] + c[2] * m[5] +
+                  c[1] * m[6] + c[0] * m[7];
     return fpv + fmv;
-  } else if (phif < PT<R>::phi2(P)) {
+  } else if (br == 1) {
     const R tmp1 = R(13.0 / 12.0), tmp2 = R(1.0 / 6.0);
     const R e = PT<R>::weps(P);
-    R V1 = fp[ic - 3 * s], V2 = fp[ic - 2 * s], V3 = fp[ic - s], V4 = fp[ic], V5 = fp[ic + s];
+    R V1 = p[1], V2 = p[2], V3 = p[3], V4 = p[4], V5 = p[5];
     R t = V1 - R(2) * V2 + V3, t2 = V1 - R(4) * V2 + R(3) * V3;
     R s11 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
     t = V2 - R(2) * V3 + V4;
@@ -170,11 +179,11 @@ __device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm
     R v3 = R(2) * V3 + R(5) * V4 - V5;
     const R fpv = tmp2 * invsum * (s11 * v1 + s22 * v2 + s33 * v3);
 
-    V1 = fm[ic - 2 * s];
-    V2 = fm[ic - s];
-    V3 = fm[ic];
-    V4 = fm[ic + s];
-    V5 = fm[ic + 2 * s];
+    V1 = m[2];
+    V2 = m[3];
+    V3 = m[4];
+    V4 = m[5];
+    V5 = m[6];
     t = V5 - R(2) * V4 + V3;
     t2 = V5 - R(4) * V4 + R(3) * V3;
     s11 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
@@ -192,10 +201,35 @@ __device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm
     const R fmv = tmp2 * invsum * (s11 * v1 + s22 * v2 + s33 * v3);
     return fpv + fmv;
   } else {
-    const R fpv = fp[ic - s] + R(0.5) * minmod(fp[ic] - fp[ic - s], fp[ic - s] - fp[ic - 2 * s]);
-    const R fmv = fm[ic] - R(0.5) * minmod(fm[ic + s] - fm[ic], fm[ic] - fm[ic - s]);
+    const R fpv = p[3] + R(0.5) * minmod(p[4] - p[3], p[3] - p[2]);
+    const R fmv = m[4] - R(0.5) * minmod(m[5] - m[4], m[4] - m[3]);
     return fpv + fmv;
   }
+}
+
+// pointer form: gathers the stencil and evaluates weno_face_v
+template <typename R>
+__device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm, long ic, long s,
+                              R phif, const DevParams* __restrict__ P) {
+  const int br = weno_branch<R>(phif, P);
+  R p[8], m[8];
+  if (br == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) p[k] = fp[ic + (k - 4) * s];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) m[k] = fm[ic + (k - 4) * s];
+  } else if (br == 1) {
+#pragma unroll
+    for (int k = 1; k < 6; ++k) p[k] = fp[ic + (k - 4) * s];
+#pragma unroll
+    for (int k = 2; k < 7; ++k) m[k] = fm[ic + (k - 4) * s];
+  } else {
+#pragma unroll
+    for (int k = 2; k < 5; ++k) p[k] = fp[ic + (k - 4) * s];
+#pragma unroll
+    for (int k = 3; k < 6; ++k) m[k] = fm[ic + (k - 4) * s];
+  }
+  return weno_face_v<R>(p, m, br, P);
 }
 
 template <typename R>
@@ -218,30 +252,119 @@ __global__ void __launch_bounds__(kBlock)
     F[f + n * nf] = weno_face<R>(Fp + (long)n * g.NP, Fm + (long)n * g.NP, ic, s, phif, P);
 }
 
+// y and z sweeps: one thread per (x, transverse) line marches through a
+// segment of faces along the sweep, holding the eight-point stencil of Fp and
+// Fm in registers, so each point is read once per segment (plus a 7-point
+// halo) with x-coalesced loads; the face branches are found once, from phi,
+// and kept as 2-bit codes (3 = tagged face, skipped) for every variable.
+constexpr int kMarchSeg = 64;
+template <typename R>
+__global__ void __launch_bounds__(64)
+    k_weno_march(R* __restrict__ F, const R* __restrict__ phi, const R* __restrict__ Fp,
+                 const R* __restrict__ Fm, int NV, const int64_t* __restrict__ tag,
+                 const DevParams* __restrict__ P, Geom g, int dir, int seg) {
+  const int a = blockIdx.x * 64 + threadIdx.x;
+  if (a >= g.Nx) return;
+  const int tr = blockIdx.y;  // z index (y sweep) or y index (z sweep)
+  const int nd = (dir == 1 ? g.Ny : g.Nz) + 1;  // faces along the sweep
+  const int j0 = blockIdx.z * seg;
+  const int j1 = min(j0 + seg, nd);
+  if (j0 >= j1) return;
+  const long s = (dir == 1) ? (long)g.Nxt : (long)g.Nxt * g.Nyt;
+  const long base = (dir == 1) ? (a + NG) + (long)g.Nxt * g.Nyt * (tr + NG)
+                               : (a + NG) + (long)g.Nxt * (tr + NG);
+  const long nf = (dir == 1) ? (long)g.Nx * nd * g.Nz : (long)g.Nx * g.Ny * nd;
+  // face j -> F index: a + Nx (j + nd tr) (y sweep) or a + Nx (tr + Ny j) (z sweep)
+  const long fbase = (dir == 1) ? a + (long)g.Nx * nd * tr : a + (long)g.Nx * tr;
+  const long fstr = (dir == 1) ? (long)g.Nx : (long)g.Nx * g.Ny;
+  unsigned long long c0 = 0, c1 = 0;  // bit j - j0 of the 2-bit branch code
+  {
+    R ph_prev = phi[base + (long)(j0 + NG - 1) * s];
+    for (int j = j0; j < j1; ++j) {
+      const long ic = base + (long)(j + NG) * s;
+      const R ph = phi[ic];
+      int br = weno_branch<R>(fmax(ph_prev, ph), P);
+      if (tag && tag[ic] == 1) br = 3;
+      c0 |= (unsigned long long)(br & 1) << (j - j0);
+      c1 |= (unsigned long long)(br >> 1) << (j - j0);
+      ph_prev = ph;
+    }
+  }
+  for (int n = 0; n < NV; ++n) {
+    const R* fp = Fp + (long)n * g.NP;
+    const R* fm = Fm + (long)n * g.NP;
+    R pw[8], mw[8];
+    const long ic0 = base + (long)(j0 + NG) * s;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      pw[k + 1] = fp[ic0 + (k - 4) * s];
+      mw[k + 1] = fm[ic0 + (k - 4) * s];
+    }
+    for (int j = j0; j < j1; ++j) {
+      const long ic = base + (long)(j + NG) * s;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        pw[k] = pw[k + 1];
+        mw[k] = mw[k + 1];
+      }
+      pw[7] = fp[ic + 3 * s];
+      mw[7] = fm[ic + 3 * s];
+      const int br = (int)((c0 >> (j - j0)) & 1ull) | ((int)((c1 >> (j - j0)) & 1ull) << 1);
+      if (br != 3) F[fbase + (long)j * fstr + (long)n * nf] = weno_face_v<R>(pw, mw, br, P);
+    }
+  }
+}
+
 // ---- launchers --------------------------------------------------------------
-int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
-                      const double* Ay, const double* Az, const int64_t* tag, hipStream_t s) {
+int launch_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const void* Ax,
+                      const void* Ay, const void* Az, const int64_t* tag, hipStream_t s,
+                      bool mf) {
   const int NP = c->g.NP;
+  const dim3 gr(nblocks(NP)), bl(kBlock);
   if (c->precision == F3D_FP64)
-    hipLaunchKernelGGL(k_flux_split<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s, Q,
-                       (double*)Fp, (double*)Fm, Ax, Ay, Az, tag, c->dparams, NP);
+    hipLaunchKernelGGL((k_flux_split<double, double>), gr, bl, 0, s, Q, (double*)Fp, (double*)Fm,
+                       (const double*)Ax, (const double*)Ay, (const double*)Az, tag, c->dparams,
+                       NP);
+  else if (mf)
+    hipLaunchKernelGGL((k_flux_split<float, float>), gr, bl, 0, s, Q, (float*)Fp, (float*)Fm,
+                       (const float*)Ax, (const float*)Ay, (const float*)Az, tag, c->dparams, NP);
   else
-    hipLaunchKernelGGL(k_flux_split<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s, Q,
-                       (float*)Fp, (float*)Fm, Ax, Ay, Az, tag, c->dparams, NP);
+    hipLaunchKernelGGL((k_flux_split<float, double>), gr, bl, 0, s, Q, (float*)Fp, (float*)Fm,
+                       (const double*)Ax, (const double*)Ay, (const double*)Az, tag, c->dparams,
+                       NP);
   return hip_check(hipGetLastError(), "flux_split");
 }
 
 int launch_species_split(f3d_ctx* c, const double* rhoi, const double* Q, void* Fp, void* Fm,
-                         const double* Ax, const double* Ay, const double* Az,
-                         const int64_t* tag, hipStream_t s) {
+                         const void* Ax, const void* Ay, const void* Az,
+                         const int64_t* tag, hipStream_t s, bool mf) {
   const int NP = c->g.NP;
+  const dim3 gr(nblocks(NP)), bl(kBlock);
   if (c->precision == F3D_FP64)
-    hipLaunchKernelGGL(k_species_split<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s, rhoi, Q,
-                       (double*)Fp, (double*)Fm, Ax, Ay, Az, tag, NP, c->g.Ns);
+    hipLaunchKernelGGL((k_species_split<double, double>), gr, bl, 0, s, rhoi, Q, (double*)Fp,
+                       (double*)Fm, (const double*)Ax, (const double*)Ay, (const double*)Az, tag,
+                       NP, c->g.Ns);
+  else if (mf)
+    hipLaunchKernelGGL((k_species_split<float, float>), gr, bl, 0, s, rhoi, Q, (float*)Fp,
+                       (float*)Fm, (const float*)Ax, (const float*)Ay, (const float*)Az, tag, NP,
+                       c->g.Ns);
   else
-    hipLaunchKernelGGL(k_species_split<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s, rhoi, Q,
-                       (float*)Fp, (float*)Fm, Ax, Ay, Az, tag, NP, c->g.Ns);
+    hipLaunchKernelGGL((k_species_split<float, double>), gr, bl, 0, s, rhoi, Q, (float*)Fp,
+                       (float*)Fm, (const double*)Ax, (const double*)Ay, (const double*)Az, tag,
+                       NP, c->g.Ns);
   return hip_check(hipGetLastError(), "species_split");
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_to_float(float* __restrict__ dst, const double* __restrict__ src, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+int launch_to_float(f3d_ctx* c, float* dst, const double* src, long n, hipStream_t s) {
+  (void)c;
+  hipLaunchKernelGGL(k_to_float, dim3(nblocks(n)), dim3(kBlock), 0, s, dst, src, n);
+  return hip_check(hipGetLastError(), "to_float");
 }
 
 int launch_shock_sensor(f3d_ctx* c, void* phi, const double* Q, hipStream_t s) {
@@ -259,14 +382,28 @@ int launch_weno(f3d_ctx* c, int dir, void* F, const void* phi, const void* Fp, c
                 int NV, const int64_t* tag, hipStream_t s) {
   const Geom& g = c->g;
   const long nf = (long)(g.Nx + (dir == 0)) * (g.Ny + (dir == 1)) * (g.Nz + (dir == 2));
-  if (c->precision == F3D_FP64)
+  if (dir != 0) {
+    const int nd = (dir == 1 ? g.Ny : g.Nz) + 1;
+    const int nseg = (nd + kMarchSeg - 1) / kMarchSeg;
+    const int seg = (nd + nseg - 1) / nseg;  // <= kMarchSeg: the branch codes fit 64 bits
+    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg);
+    if (c->precision == F3D_FP64)
+      hipLaunchKernelGGL(k_weno_march<double>, grid, dim3(64), 0, s, (double*)F,
+                         (const double*)phi, (const double*)Fp, (const double*)Fm, NV, tag,
+                         c->dparams, g, dir, seg);
+    else
+      hipLaunchKernelGGL(k_weno_march<float>, grid, dim3(64), 0, s, (float*)F,
+                         (const float*)phi, (const float*)Fp, (const float*)Fm, NV, tag,
+                         c->dparams, g, dir, seg);
+  } else if (c->precision == F3D_FP64) {
     hipLaunchKernelGGL(k_weno<double>, dim3(nblocks(nf)), dim3(kBlock), 0, s, (double*)F,
                        (const double*)phi, (const double*)Fp, (const double*)Fm, NV, tag,
                        c->dparams, g, dir);
-  else
+  } else {
     hipLaunchKernelGGL(k_weno<float>, dim3(nblocks(nf)), dim3(kBlock), 0, s, (float*)F,
                        (const float*)phi, (const float*)Fp, (const float*)Fm, NV, tag,
                        c->dparams, g, dir);
+  }
   return hip_check(hipGetLastError(), "weno");
 }
 

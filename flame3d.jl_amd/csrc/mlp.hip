@@ -351,13 +351,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 // Fused surrogate chemistry step (pre_input -> network -> post_predict,
 // reactions.jl:2-56): one wave = 64 interior cells.  The network runs for all
 // 64 lanes (uniform control flow around the MFMAs); each active lane then
-// applies four gates and either commits the prediction, with the FP64 energy
+// applies five gates and either commits the prediction, with the FP64 energy
 // bookkeeping of the backward-Euler integrator (chem_epilogue), or appends its
-// cell to the fallback list and leaves it untouched:
+// cell to the fallback list and leaves it untouched (status = 1 + gate):
 //   1. Mahalanobis distance of the normalized inputs <= d_max (out of distribution);
 //   2. element balance of the raw prediction (then projected to conserve elements);
 //   3. positivity, Y_k + dY_k >= -neg_tol;
-//   4. backward-Euler residual at the predicted state, max_k |R_k| / rho <= res_tol,
+//   4. predicted temperature change sum_k |h_f,k dY_k| / c_v <= dT_max (fast
+//      chemistry, where a relative error of the network costs the most);
+//   5. backward-Euler residual at the predicted state, max_k |R_k| / rho <= res_tol
+//      and sum_k |h_f,k R_k| / (rho c_v) <= res_T_tol,
 //      R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T'), T' from the energy (error gate).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
@@ -408,7 +411,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? xn[k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
-  int code = act ? 1 : 0;  // 1 accepted, 2..5 the first gate that failed
+  int code = act ? 1 : 0;  // 1 accepted, 2..6 the first gate that failed
   if (act) {
     // 1. Mahalanobis distance of the normalized inputs
     float d2 = 0.0f;
@@ -461,6 +464,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       if (Y[n] + dY[n] < -(double)C->neg_tol) pok = false;
     }
     if (code == 1 && !pok) code = 4;
+    // 4. fast chemistry: the temperature change the prediction implies
+    if (code == 1 && C->dT_max > 0.0f) {
+      double esum = 0.0;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * dY[n]);
+      if (!(esum * rho <= (double)C->dT_max * cv_mix<NS>(P, T, r))) code = 5;
+    }
     if (code == 1) {
       double rn[NS], dei = 0.0, sr = 0.0;
 #pragma unroll
@@ -476,17 +486,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       for (int n = 0; n < NS; ++n) rn[n] *= rhoc / sr;
       const double ein = Q[p + 6 * NP] + dei;
       const double Tn = fmax(get_T<NS>(P, ein, rn, nullptr), kEps);
-      // 4. backward-Euler residual at (rho', T')
-      if (C->res_tol > 0.0f) {
+      // 5. backward-Euler residual at (rho', T'): its largest species entry,
+      //    and the temperature error its formation energies imply
+      if (C->res_tol > 0.0f || C->res_T_tol > 0.0f) {
         double sc[NS], wdot[NS], Adum[NS][NS];
 #pragma unroll
         for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
         production_rate<NS, false>(P, sc, Tn, Adum, wdot);
-        double rmax = 0.0;
+        double rmax = 0.0, esum = 0.0;
 #pragma unroll
-        for (int n = 0; n < NS; ++n)
-          rmax = fmax(rmax, fabs(rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt));
-        if (!(rmax <= (double)C->res_tol * rhoc)) code = 5;
+        for (int n = 0; n < NS; ++n) {
+          const double R = rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt;
+          rmax = fmax(rmax, fabs(R));
+          esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * R);
+        }
+        if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
+        if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cv_mix<NS>(P, Tn, rn)))
+          code = 6;
       }
       if (code == 1) {
         U[p + 4 * NP] += dei;
@@ -501,7 +517,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   }
   const bool ok = (code == 1);
   // wave-aggregated append of the rejected cells
-  const bool rej = act && !ok;  // code 2..5
+  const bool rej = act && !ok;  // code 2..6
   const unsigned long long mask = __ballot(rej);
   if (mask) {
     const int leader = __ffsll((long long)mask) - 1;
@@ -632,6 +648,8 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   h.elem_tol = cfg->elem_tol;
   h.neg_tol = cfg->neg_tol;
   h.res_tol = cfg->res_tol;
+  h.res_T_tol = cfg->res_T_tol;
+  h.dT_max = cfg->dT_max;
   h.boxcox_lambda = cfg->boxcox_lambda;
   h.label_mode = cfg->label_mode;
   for (int k = 0; k < NN_OUT; ++k) h.label_scale[k] = cfg->label_scale[k];

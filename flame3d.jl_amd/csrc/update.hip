@@ -151,16 +151,21 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
       phi[n][nn] = tmp;
     }
   }
+  // D_k = (1 - X_k) / (P sum_{l!=k} X_l / D_kl + eps) with 1 - X_k taken as
+  // sum_{l!=k} X_l (equal since sum X = 1): the difference 1 - X_k cancels
+  // catastrophically in nearly pure cells, where the sum stays exact (a
+  // binary mixture then gives D_1 = D_12 / P for any composition)
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
-    R s = 0;
+    R s = 0, sx = 0;
 #pragma unroll
     for (int nn = 0; nn < NS; ++nn) {
       if (nn == n) continue;
       s += X[nn] / phi[n][nn];
+      sx += X[nn];
     }
     s *= Pr;
-    Diff[n] = (R(1) - X[n]) / (s + PT<R>::eps());
+    Diff[n] = sx / (s + PT<R>::eps());
   }
 }
 

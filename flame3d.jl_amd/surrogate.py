@@ -163,8 +163,9 @@ LABEL_SCALE = 1e-9  # asinh label scale: dY below it is linear, above it logarit
 
 
 def make_config(mech, data: Dataset, label_mode: int = 2, label_scale: float = LABEL_SCALE,
-                elem_tol: float = 0.05, neg_tol: float = 1e-7, res_tol: float = 0.0,
-                sigmas: float = 3.0, ridge: float = 1e-6) -> abi.NNConfig:
+                elem_tol: float = 0.05, neg_tol: float = 1e-7, res_tol: float = 3e-5,
+                res_T_tol: float = 0.25, dT_max: float = 20.0, sigmas: float = 3.0,
+                ridge: float = 1e-6) -> abi.NNConfig:
     """Normalization and gate statistics from the training data.  The
     Mahalanobis threshold is the training distances' quantile at the one-sided
     `sigmas`-sigma coverage of a normal law (0.99865 for 3 sigma)."""
@@ -199,6 +200,8 @@ def make_config(mech, data: Dataset, label_mode: int = 2, label_scale: float = L
     cfg.elem_tol = elem_tol
     cfg.neg_tol = neg_tol
     cfg.res_tol = res_tol
+    cfg.res_T_tol = res_T_tol
+    cfg.dT_max = dT_max
     cfg.boxcox_lambda = LAMBDA
     cfg.label_mode = label_mode
     for k in range(NN_OUT):
@@ -263,8 +266,8 @@ def mse(ctx: abi.Context, Xn, Yn) -> float:
 # ---- persistence (plain arrays: np.load with allow_pickle=False) ----------
 _CFG_ARRAYS = ("in_mean", "in_std", "out_mean", "out_std", "mahal_mean", "mahal_inv",
                "label_scale")
-_CFG_SCALARS = ("d_max", "elem_tol", "neg_tol", "res_tol", "boxcox_lambda", "label_mode",
-                "n_elem", "dt")
+_CFG_SCALARS = ("d_max", "elem_tol", "neg_tol", "res_tol", "res_T_tol", "dT_max",
+                "boxcox_lambda", "label_mode", "n_elem", "dt")
 
 
 def save(path: str, cfg: abi.NNConfig, params: np.ndarray, meta: Optional[Dict] = None):
@@ -289,7 +292,8 @@ def load(path: str = DEFAULT_PATH):
         for k in range(NN_OUT):
             cfg.elem[e][k] = float(z["elem"][e, k])
     for k in _CFG_SCALARS:
-        setattr(cfg, k, type(getattr(cfg, k))(z[k][()]))
+        if k in z.files:  # gates added later default to off
+            setattr(cfg, k, type(getattr(cfg, k))(z[k][()]))
     return cfg, z["params"].astype(np.float32), json.loads(str(z["meta"][()]))
 
 

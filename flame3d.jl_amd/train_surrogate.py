@@ -1,7 +1,7 @@
 """Train the H2/O2 chemistry surrogate on the GPU and save it.
 
     python flame3d.jl_amd/train_surrogate.py --out PATH [--samples-shape 64 64 32]
-        [--steps 2000] [--records 32] [--epochs 60] [--batch 8192] [--lr 2e-3]
+        [--steps 2000] [--records 32] [--epochs 600] [--batch 8192] [--lr 2e-3]
 
 Data: trajectories of the library's FP64 backward-Euler Newton integrator
 (f3d_chem_newton, dt = 1e-7 s) from H2/O2/N2 reactors with T ~ U[900, 2500] K,
@@ -73,11 +73,13 @@ def main(argv=None):
     ap.add_argument("--samples-shape", type=int, nargs=3, default=(64, 64, 32))
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--records", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--decay-every", type=int, default=20)
-    ap.add_argument("--res-tol", type=float, default=2e-6)
+    ap.add_argument("--decay-every", type=int, default=200)
+    ap.add_argument("--res-tol", type=float, default=3e-5)
+    ap.add_argument("--res-T-tol", type=float, default=0.25)
+    ap.add_argument("--dT-max", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=12345)
     a = ap.parse_args(argv)
 
@@ -98,7 +100,7 @@ def main(argv=None):
     t_gen = time.time() - t0
     M = data.X.shape[0]
     log(f"  {M} samples in {t_gen:.1f}s, mean Newton iterations {data.newton_iters:.2f}")
-    cfg = sur.make_config(mech, data, res_tol=a.res_tol)
+    cfg = sur.make_config(mech, data, res_tol=a.res_tol, res_T_tol=a.res_T_tol, dT_max=a.dT_max)
     Xn, Yn = sur.normalize(cfg, data)
     del data
     # hold out 1/16 for validation

@@ -192,6 +192,10 @@ typedef struct {
   float neg_tol;         /* Y_k + dY_k < -neg_tol falls back */
   float res_tol;         /* backward-Euler residual max_k |R_k|/rho > res_tol falls back
                             (R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T')); <= 0: off */
+  float res_T_tol;       /* ... or sum_k |h_f,k R_k| / (rho c_v) > res_T_tol [K], the
+                            temperature error those residuals imply; <= 0: off */
+  float dT_max;          /* predicted chemical temperature change sum_k |h_f,k dY_k| / c_v
+                            above dT_max [K] falls back (fast chemistry); <= 0: off */
   float boxcox_lambda;
   int32_t label_mode;    /* outputs: 0 dY_k; 1 d BoxCox(Y_k) (post_predict form,
                             Y' = (lambda (BoxCox(Y) + out) + 1)^(1/lambda));
@@ -225,7 +229,8 @@ int f3d_nn_forward(f3d_ctx* ctx, const float* X, float* Y, int32_t B, hipStream_
  * integrated by the FP64 backward-Euler Newton (f3d_chem_newton settings).
  * status [Nx*Ny*Nz] (device, may be NULL): 0 untouched, 1 surrogate; Newton
  * fallback after the gate that rejected the cell: 2 Mahalanobis distance,
- * 3 element balance, 4 positivity, 5 backward-Euler residual.
+ * 3 element balance, 4 positivity, 5 predicted temperature change,
+ * 6 backward-Euler residual.
  * *n_fallback (host) receives the number of fallback cells; this call
  * synchronizes the stream to size the fallback launch. */
 int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int32_t maxiter,
@@ -234,7 +239,10 @@ int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int
 
 /* ---- stage level -------------------------------------------------------- */
 /* Registers the block's metrics (9 device pointers, order as above) and J;
- * the context keeps the pointers (and, for F3D_FP32, float copies). */
+ * the context keeps the pointers and, for F3D_FP32, float copies of the
+ * arrays, which the FP32 stage and the kernel-granular split / viscous calls
+ * given these same pointers read instead (the values the kernels would
+ * convert to anyway).  Call again after changing the arrays' contents. */
 int f3d_set_metrics(f3d_ctx* ctx, const double* const* metrics, const double* J);
 /* One Shu-Osher RK3 stage KRK = 1, 2, 3 exactly as solver.jl:298-326:
  * (KRK == 1: Un <- U, rhon <- rhoi), mixture, shockSensor, specAdvance,
@@ -242,6 +250,20 @@ int f3d_set_metrics(f3d_ctx* ctx, const double* const* metrics, const double* J)
  * context-owned device memory (allocated on first use). */
 int f3d_rk3_stage(f3d_ctx* ctx, int32_t KRK, double* U, double* Un, double* rhoi, double* rhon,
                   double* Q, double dt, const int64_t* tag, hipStream_t s);
+/* Chemistry of f3d_time_step's Strang half-steps. */
+#define F3D_CHEM_NONE 0
+#define F3D_CHEM_STIFF 1     /* f3d_chem_stiff, sub_step substeps (eval_gpu_stiff) */
+#define F3D_CHEM_EXPLICIT 2  /* f3d_chem_explicit, sub_step substeps (eval_gpu) */
+#define F3D_CHEM_NEWTON 3    /* f3d_chem_newton (maxiter, rtol, ttol) */
+#define F3D_CHEM_SURROGATE 4 /* f3d_nn_chem; *n_fallback accumulates the fallbacks */
+/* One time step as solver.jl:240-378: chemistry over dt/2 and f3d_fill,
+ * the three f3d_rk3_stage calls, chemistry over dt/2 and f3d_fill. */
+int f3d_time_step(f3d_ctx* ctx, double* U, double* Un, double* rhoi, double* rhon, double* Q,
+                  double dt, int32_t chem, int32_t sub_step, int32_t maxiter, double rtol,
+                  double ttol, int64_t* n_fallback, const int64_t* tag, hipStream_t s);
+/* any(isnan, U) of solver.jl:388 over the 5 padded fields of U; *flag (host)
+ * = 1 if a NaN is present, else 0.  Synchronizes the stream. */
+int f3d_nan_check(f3d_ctx* ctx, const double* U, int32_t* flag, hipStream_t s);
 /* Device pointer of a context-owned intermediate after f3d_rk3_stage, by
  * name ("Yi", "lambda", "mu", "D", "phi", "Fx", "Fy", "Fz", "Fvx", ...,
  * "Fhz"); NULL if unknown. */

@@ -275,14 +275,18 @@ void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, doub
       D[n * Ns + nn] = tmp;
     }
   }
+  // thermo.jl:359-368 with 1 - X_k evaluated as sum_{l!=k} X_l (equal in
+  // exact arithmetic since sum X = 1; the difference cancels catastrophically
+  // in nearly pure cells, where it is rounding noise)
   for (int n = 0; n < Ns; ++n) {
-    double s = 0.0;
+    double s = 0.0, sx = 0.0;
     for (int nn = 0; nn < Ns; ++nn) {
       if (nn == n) continue;
       s += X[nn] / D[n * Ns + nn];
+      sx += X[nn];
     }
     s *= P;
-    Diff[n] = (1 - X[n]) / (s + kEps);
+    Diff[n] = sx / (s + kEps);
   }
   *lambda_out = lambda;
   *mu_out = mu;

@@ -302,10 +302,10 @@ def test_nn_chem_residual_gate():
     torch.cuda.synchronize()
     s = st.cpu().numpy().reshape(-1)
     assert nfb == int((s >= 2).sum()) and 0 < nfb < s.size
-    assert set(np.unique(s).tolist()) <= {1, 5}  # only the residual gate is active
+    assert set(np.unique(s).tolist()) <= {1, 6}  # only the residual gate is active
     q1, r1, qn, rnn = _interior(Q), _interior(r), _interior(Qn), _interior(rn)
     # rejected cells are exactly the Newton result
-    rej = s == 5
+    rej = s == 6
     assert np.array_equal(q1[rej], qn[rej]) and np.array_equal(r1[rej], rnn[rej])
     orc = Oracle(*SIZE, m)
     mw = np.asarray(m.mw)

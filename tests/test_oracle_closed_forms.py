@@ -484,6 +484,29 @@ def test_mixture_single_and_identical_species(air):
     assert abs(mu2 - mu) < 1e-12 * mu and abs(lam2 - lam) < 1e-12 * lam
 
 
+def test_mixture_diffusivity_binary_limit(air):
+    """Mixture-averaged D_k = (1 - X_k) / (P sum_{l != k} X_l / D_kl + eps):
+    in a binary mixture D_1 = x_2 / (P x_2 / D_12 + eps) (-> D_12 / P) for
+    every composition, including the nearly pure limits where 1 - X_1 is
+    computed as sum_{l != 1} X_l (a plain 1 - X_1 would cancel to rounding
+    noise)."""
+    from oracle import Oracle
+    orc = Oracle(1, 1, 1, air)
+    T, P = 1500.0, 2e5
+    i, j = air.index("N2"), air.index("O2")
+    lgT = math.log(T)
+    D12 = T ** 1.5 * np.polyval(air.binarydiff_poly[::-1, i, j], lgT)
+    for x2 in (0.5, 1e-3, 1e-9, 1e-14):
+        X = np.zeros(5)
+        X[i], X[j] = 1.0 - x2, x2
+        _, _, D = orc.mixture_properties(T, P, X)
+        eps = 2.220446049250313e-16
+        for k, xo in ((i, x2), (j, 1.0 - x2)):
+            exp = xo / (P * xo / D12 + eps)
+            assert abs(D[k] - exp) < 1e-12 * exp, (x2, k, D[k], exp)
+        assert abs(D[j] * P - D12) < 1e-9 * D12
+
+
 def test_bug_compat_flag_reads_transposed_coefficients(h2):
     """eval_gpu_stiff's coeffs_lo[n, 6] (reactions.jl:282) reads linear index
     35 + n of the 7 x Ns array; with the flag the energy change differs."""
