@@ -2,7 +2,8 @@
 FP32 flux + learned chemistry vs the plain FP64 method, on one MI355X.
 
 Workload: the reacting H2/air shock-laden mixing layer on a 256^3 curvilinear
-block (tanh-stretched y, factor 2.5): 9-species H2/O2 mechanism, hybrid UP7 /
+block (tanh-stretched y, factor 2.5; Ma 1.5 air at 1200 K over H2 at 1000 K,
+3 atm, normal shock): 9-species H2/O2 mechanism, hybrid UP7 /
 WENO5-Z / NND with the pressure sensor, viscous + species diffusion, and
 Strang-split chemistry (solver.jl:240-378): per time step, chemistry over
 dt/2, the three RK3 stages, chemistry over dt/2 (dt = 2e-7 s at CFL 0.5).

@@ -34,18 +34,19 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
   for (int k = 0; k < maxiter; ++k) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) sc[n] = r[n] / P->mw[n] * 1e-6;
-    double Ar[NS][NS], wdot2[NS];
-    production_rate<NS, true, true>(P, sc, T, Ar, wdot);
+    // the rate Jacobian is built in the top-left block of the system matrix
+    // and turned into I - W_n/W_l h dwdot/dc in place (one matrix in registers)
+    double A[N1][N1], wdot2[NS], rhs[N1], dx[N1];
+    production_rate<NS, true, true, N1>(P, sc, T, A, wdot);
     const double hT = 1e-6 * T;
-    production_rate<NS, false>(P, sc, T + hT, Ar, wdot2);  // Ar untouched (JAC=false)
-    double A[N1][N1], rhs[N1], dx[N1];
+    production_rate<NS, false, false, N1>(P, sc, T + hT, A, wdot2);  // A untouched (JAC=false)
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const double src = wdot[n] * P->mw[n] * 1e6 * h;
       rhs[n] = -(r[n] - rs[n] - src);
 #pragma unroll
       for (int l = 0; l < NS; ++l)
-        A[n][l] = (n == l ? 1.0 : 0.0) - Ar[n][l] * P->mw[n] / P->mw[l] * h;
+        A[n][l] = (n == l ? 1.0 : 0.0) - A[n][l] * P->mw[n] / P->mw[l] * h;
       A[n][NS] = -((wdot2[n] - wdot[n]) / hT) * P->mw[n] * 1e6 * h;
     }
     double E = internal_energy<NS>(P, T, r);
@@ -98,7 +99,8 @@ __global__ void __launch_bounds__(kBlock)
            int maxiter, double rtol, double ttol, int32_t* __restrict__ iters,
            const int64_t* __restrict__ tag, const DevParams* __restrict__ P, Geom g, int bug,
            const int32_t* __restrict__ list, long nwork) {
-  // work item -> interior cell id (all cells, or the cells of a compacted list)
+  // one thread per interior cell, or per entry of an ordered cell list (the
+  // surrogate's rejected cells)
   const long t_ = gid();
   if (t_ >= nwork) return;
   const long id = list ? (long)list[t_] : t_;

@@ -40,9 +40,11 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 // Arate[l, m] = d wdot_l / d c_m; the reference neglects the third-body and
 // T dependence, FULL (Newton mode) adds the [M] derivative of three-body and
 // falloff rates (Lindemann part).
-template <int NS, bool JAC, bool FULL = false>
+// A is any matrix with at least NS x NS entries (the Newton solve passes its
+// (NS+1) x (NS+1) system so the rate Jacobian is built in place).
+template <int NS, bool JAC, bool FULL = false, int NA = NS>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
-                                       double (&A)[NS][NS], double* wdot) {
+                                       double (&A)[NA][NA], double* wdot) {
   const double lgT = log(T);
   const double invT = 1.0 / T;
   double gi[NS];

@@ -79,8 +79,10 @@ struct f3d_ctx {
   void* nn_cfg = nullptr;  // device NNDev
   double nn_dt = 0;
   long nn_step = 0;
-  int32_t* nn_fb = nullptr;  // fallback cell list [Nx*Ny*Nz] and its counter
-  int32_t* nn_fbcount = nullptr;
+  int8_t* nn_status = nullptr;  // per-cell gate codes [Nx*Ny*Nz] when the caller passes none
+  int32_t* nn_fbcount = nullptr;  // rejected cells per 64-cell tile, then their offsets
+  int32_t* nn_fblist = nullptr;   // rejected cell ids in cell order [Nx*Ny*Nz]
+  int32_t* nn_fbtotal = nullptr;  // number of rejected cells
 };
 
 namespace f3d {

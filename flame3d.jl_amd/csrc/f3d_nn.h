@@ -53,11 +53,12 @@ struct NNDev {
   float mahal_mean[NN_IN];
   float mahal_inv[NN_IN][NN_IN];  // inverse covariance of the normalized inputs
   float d_max;                     // Mahalanobis-distance gate
-  float elem_tol;                  // element-balance gate (relative)
+  float elem_tol;                  // element-balance gate (net / gross turnover)
   float neg_tol;                   // most negative admissible Y_k + dY_k
   float res_tol;                   // backward-Euler residual gate (<= 0: off)
   float res_T_tol;                 // temperature-weighted residual gate [K] (<= 0: off)
   float dT_max;                    // predicted chemical temperature change gate [K] (<= 0: off)
+  float chi_max;                   // thermal explosiveness gate (<= 0: off)
   float boxcox_lambda;             // Y -> (Y^lambda - 1)/lambda input transform
   int label_mode;                  // 0: dY; 1: d BoxCox(Y); 2: asinh(dY / label_scale)
   float label_scale[NN_OUT];

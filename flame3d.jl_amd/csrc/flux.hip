@@ -151,8 +151,7 @@ __device__ inline int weno_branch(R phif, const DevParams* __restrict__ P) {
 // values v[k] = f[ic + (k - 4) s] of Fp (p) and Fm (m) around the face i-1/2
 // (ic its right cell, s the sweep stride).
 template <typename R>
-__device__ inline R weno_face_v(const R (&p)[8], const R (&m)[8], int br,
-                                const DevParams* __restrict__ P) {
+__device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams* __restrict__ P) {
   const R* c = PT<R>::up7(P);
   if (br == 0) {
     const R fpv = c[0] * p[0] + c[1] * p[1] + c[2] * p[2] + c[3] * p[3] + c[4] * p[4] +
@@ -257,7 +256,7 @@ __global__ void __launch_bounds__(kBlock)
 // Fm in registers, so each point is read once per segment (plus a 7-point
 // halo) with x-coalesced loads; the face branches are found once, from phi,
 // and kept as 2-bit codes (3 = tagged face, skipped) for every variable.
-constexpr int kMarchSeg = 64;
+constexpr int kMarchSeg = 32;
 template <typename R>
 __global__ void __launch_bounds__(64)
     k_weno_march(R* __restrict__ F, const R* __restrict__ phi, const R* __restrict__ Fp,
@@ -280,6 +279,7 @@ __global__ void __launch_bounds__(64)
   unsigned long long c0 = 0, c1 = 0;  // bit j - j0 of the 2-bit branch code
   {
     R ph_prev = phi[base + (long)(j0 + NG - 1) * s];
+#pragma unroll 8
     for (int j = j0; j < j1; ++j) {
       const long ic = base + (long)(j + NG) * s;
       const R ph = phi[ic];
@@ -290,27 +290,42 @@ __global__ void __launch_bounds__(64)
       ph_prev = ph;
     }
   }
+  // faces in chunks of kCh: the kCh new stencil rows of a chunk are loaded
+  // together (independent loads in flight), rows past the array clamped
+  constexpr int kCh = 8;
+  const long rmax = base + (long)((dir == 1 ? g.Nyt : g.Nzt) - 1) * s;
   for (int n = 0; n < NV; ++n) {
     const R* fp = Fp + (long)n * g.NP;
     const R* fm = Fm + (long)n * g.NP;
-    R pw[8], mw[8];
+    R pw[kCh + 7], mw[kCh + 7];  // rows j-4 .. j+kCh+2 of the chunk starting at face j
     const long ic0 = base + (long)(j0 + NG) * s;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      pw[k + 1] = fp[ic0 + (k - 4) * s];
-      mw[k + 1] = fm[ic0 + (k - 4) * s];
+      pw[kCh + k] = fp[ic0 + (k - 4) * s];
+      mw[kCh + k] = fm[ic0 + (k - 4) * s];
     }
-    for (int j = j0; j < j1; ++j) {
+    for (int j = j0; j < j1; j += kCh) {
       const long ic = base + (long)(j + NG) * s;
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
-        pw[k] = pw[k + 1];
-        mw[k] = mw[k + 1];
+        pw[k] = pw[kCh + k];
+        mw[k] = mw[kCh + k];
       }
-      pw[7] = fp[ic + 3 * s];
-      mw[7] = fm[ic + 3 * s];
-      const int br = (int)((c0 >> (j - j0)) & 1ull) | ((int)((c1 >> (j - j0)) & 1ull) << 1);
-      if (br != 3) F[fbase + (long)j * fstr + (long)n * nf] = weno_face_v<R>(pw, mw, br, P);
+#pragma unroll
+      for (int k = 0; k < kCh; ++k) {
+        const long q = min(ic + (long)(k + 3) * s, rmax);
+        pw[7 + k] = fp[q];
+        mw[7 + k] = fm[q];
+      }
+#pragma unroll
+      for (int k = 0; k < kCh; ++k) {
+        const int jj = j + k;
+        if (jj < j1) {
+          const int br = (int)((c0 >> (jj - j0)) & 1ull) | ((int)((c1 >> (jj - j0)) & 1ull) << 1);
+          if (br != 3)
+            F[fbase + (long)jj * fstr + (long)n * nf] = weno_face_v<R>(&pw[k], &mw[k], br, P);
+        }
+      }
     }
   }
 }

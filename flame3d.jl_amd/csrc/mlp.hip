@@ -351,29 +351,35 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 // Fused surrogate chemistry step (pre_input -> network -> post_predict,
 // reactions.jl:2-56): one wave = 64 interior cells.  The network runs for all
 // 64 lanes (uniform control flow around the MFMAs); each active lane then
-// applies five gates and either commits the prediction, with the FP64 energy
-// bookkeeping of the backward-Euler integrator (chem_epilogue), or appends its
-// cell to the fallback list and leaves it untouched (status = 1 + gate):
+// applies six gates and either commits the prediction, with the FP64 energy
+// bookkeeping of the backward-Euler integrator (chem_epilogue), or marks its
+// cell for the Newton pass and leaves it untouched (status = 1 + gate); the
+// marked cells are then listed in cell order (k_nn_scan, k_nn_compact):
 //   1. Mahalanobis distance of the normalized inputs <= d_max (out of distribution);
-//   2. element balance of the raw prediction (then projected to conserve elements);
+//   2. element balance of the raw prediction, each element's net change against
+//      the gross elemental turnover (then projected to conserve elements);
 //   3. positivity, Y_k + dY_k >= -neg_tol;
 //   4. predicted temperature change sum_k |h_f,k dY_k| / c_v <= dT_max (fast
 //      chemistry, where a relative error of the network costs the most);
 //   5. backward-Euler residual at the predicted state, max_k |R_k| / rho <= res_tol
 //      and sum_k |h_f,k R_k| / (rho c_v) <= res_T_tol,
-//      R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T'), T' from the energy (error gate).
+//      R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T'), T' from the energy (error gate);
+//   6. thermal explosiveness chi = dt (d HRR / dT) / (rho c_v) <= chi_max at the
+//      predicted state: near chi ~ 1 the backward-Euler equation can have a
+//      second (ignited) root that a small residual does not rule out.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_nn_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
               const int64_t* __restrict__ tag, const DevParams* __restrict__ P,
               const NNDev* __restrict__ C, const __bf16* __restrict__ Wb,
-              const float* __restrict__ master, Geom g, double dt, int32_t* __restrict__ fb_list,
-              int32_t* __restrict__ fb_count, int8_t* __restrict__ status) {
+              const float* __restrict__ master, Geom g, double dt,
+              int32_t* __restrict__ tile_count, int8_t* __restrict__ status) {
   constexpr int NS = NN_OUT;
   __shared__ __align__(16) FwdLds L;
   const int lane = threadIdx.x;
   const long ncell = (long)g.Nx * g.Ny * g.Nz;
-  const long id = (long)xcd_block() * 64 + lane;
+  const unsigned tile = xcd_block();
+  const long id = (long)tile * 64 + lane;
   const long NP = g.NP;
   long p = 0;
   bool act = false;
@@ -388,7 +394,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
     T = Q[p + 5 * NP];
     act = (!tag || tag[p] == 0) && (T > P->T_criteria);
-    if (status) status[id] = 0;
+    status[id] = 0;
   }
   if (act) {
     // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
@@ -411,7 +417,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? xn[k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
-  int code = act ? 1 : 0;  // 1 accepted, 2..6 the first gate that failed
+  int code = act ? 1 : 0;  // 1 accepted, 2..7 the first gate that failed
   if (act) {
     // 1. Mahalanobis distance of the normalized inputs
     float d2 = 0.0f;
@@ -440,17 +446,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         dY0[n] = (base > 0.0 ? pow(base, 1.0 / lam) : 0.0) - Y[n];
       }
     }
-    // 2. element balance of the raw prediction, then the projection
+    // 2. element balance of the raw prediction (net change of each element
+    //    against the gross turnover of all elements, sum_e sum_k a_ek |dY_k| / W_k,
+    //    so an inert species' noise counts against the whole change), then
+    //    the projection
     bool eok = true;
-    for (int e = 0; e < C->n_elem; ++e) {
-      double num = 0.0, den = 0.0;
+    {
+      double gross = 0.0, worst = 0.0;
+      for (int e = 0; e < C->n_elem; ++e) {
+        double num = 0.0;
 #pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        const double ae = (double)C->elem[e][n] / P->mw[n];
-        num += ae * dY0[n];
-        den += ae * Y[n];
+        for (int n = 0; n < NS; ++n) {
+          const double ae = (double)C->elem[e][n] / P->mw[n];
+          num += ae * dY0[n];
+          gross += ae * fabs(dY0[n]);
+        }
+        worst = fmax(worst, fabs(num));
       }
-      if (den > 0.0 && fabs(num) > (double)C->elem_tol * den) eok = false;
+      if (gross > 0.0 && worst > (double)C->elem_tol * gross) eok = false;
     }
     if (code == 1 && !eok) code = 3;
     bool pok = true;
@@ -488,21 +501,35 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       const double Tn = fmax(get_T<NS>(P, ein, rn, nullptr), kEps);
       // 5. backward-Euler residual at (rho', T'): its largest species entry,
       //    and the temperature error its formation energies imply
-      if (C->res_tol > 0.0f || C->res_T_tol > 0.0f) {
+      const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;
+      if (res_on || C->chi_max > 0.0f) {
         double sc[NS], wdot[NS], Adum[NS][NS];
 #pragma unroll
         for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
         production_rate<NS, false>(P, sc, Tn, Adum, wdot);
-        double rmax = 0.0, esum = 0.0;
+        const double cvn = cv_mix<NS>(P, Tn, rn);
+        if (res_on) {
+          double rmax = 0.0, esum = 0.0;
 #pragma unroll
-        for (int n = 0; n < NS; ++n) {
-          const double R = rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt;
-          rmax = fmax(rmax, fabs(R));
-          esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * R);
+          for (int n = 0; n < NS; ++n) {
+            const double R = rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt;
+            rmax = fmax(rmax, fabs(R));
+            esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * R);
+          }
+          if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
+          if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cvn)) code = 6;
         }
-        if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
-        if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cv_mix<NS>(P, Tn, rn)))
-          code = 6;
+        // 6. thermal explosiveness: the heat-release response to T over one step
+        if (code == 1 && C->chi_max > 0.0f) {
+          double w2[NS];
+          const double hT = 1e-4 * Tn;
+          production_rate<NS, false>(P, sc, Tn + hT, Adum, w2);
+          double dq = 0.0;
+#pragma unroll
+          for (int n = 0; n < NS; ++n)
+            dq -= P->lo[n][5] * P->Ru * 1e6 * (w2[n] - wdot[n]);  // h_f,k W_k d(wdot_k)
+          if (!(dt * dq / hT <= (double)C->chi_max * cvn)) code = 7;
+        }
       }
       if (code == 1) {
         U[p + 4 * NP] += dei;
@@ -513,19 +540,52 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = rn[n];
       }
     }
-    if (status) status[id] = (int8_t)code;
+    status[id] = (int8_t)code;
   }
   const bool ok = (code == 1);
-  // wave-aggregated append of the rejected cells
-  const bool rej = act && !ok;  // code 2..6
-  const unsigned long long mask = __ballot(rej);
-  if (mask) {
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
-    base = __shfl(base, leader);
-    if (rej) fb_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)id;
+  // rejected cells of this 64-cell tile (status >= 2); the ordered list the
+  // Newton pass runs over is built from these counts (k_nn_scan, k_nn_compact)
+  const bool rej = act && !ok;  // code 2..7
+  const unsigned long long bal = __ballot(rej);
+  if (lane == 0) tile_count[tile] = (int32_t)__popcll(bal);
+}
+
+// Exclusive scan of the per-tile rejected counts (one workgroup of 1024
+// threads, each over a contiguous chunk), in place; *total = their sum.
+__global__ void __launch_bounds__(1024) k_nn_scan(int32_t* __restrict__ cnt, long n,
+                                                   int32_t* __restrict__ total) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const long chunk = (n + 1023) / 1024;
+  const long b = t * chunk, e = min(b + chunk, n);
+  int32_t s = 0;
+  for (long i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int32_t v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
+  int32_t run = part[t] - s;  // exclusive prefix of this chunk
+  for (long i = b; i < e; ++i) {
+    const int32_t c = cnt[i];
+    cnt[i] = run;
+    run += c;
+  }
+  if (t == 1023) *total = part[1023];
+}
+
+// Rejected cell ids of each 64-cell tile at the tile's offset, in cell order.
+__global__ void __launch_bounds__(64)
+    k_nn_compact(const int8_t* __restrict__ status, const int32_t* __restrict__ off, long ncell,
+                 int32_t* __restrict__ list) {
+  const int lane = threadIdx.x;
+  const long id = (long)blockIdx.x * 64 + lane;
+  const bool rej = id < ncell && status[id] >= 2;
+  const unsigned long long bal = __ballot(rej);
+  if (rej) list[off[blockIdx.x] + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)id;
 }
 
 // P = I - S E^T (E S E^T)^-1 E with E[e][k] = atoms_ek / W_k and S = diag(out_std^2):
@@ -601,7 +661,7 @@ static int nn_partials(f3d_ctx* c, long tiles) {
 
 void nn_free(f3d_ctx* c) {
   void* ptrs[] = {c->nn_master, c->nn_m1, c->nn_m2, c->nn_Wb, c->nn_Wtb, c->nn_partial,
-                  c->nn_cfg, c->nn_fb, c->nn_fbcount};
+                  c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   c->nn_ready = false;
@@ -650,6 +710,7 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   h.res_tol = cfg->res_tol;
   h.res_T_tol = cfg->res_T_tol;
   h.dT_max = cfg->dT_max;
+  h.chi_max = cfg->chi_max;
   h.boxcox_lambda = cfg->boxcox_lambda;
   h.label_mode = cfg->label_mode;
   for (int k = 0; k < NN_OUT; ++k) h.label_scale[k] = cfg->label_scale[k];
@@ -669,8 +730,10 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
                hipMalloc(&c->nn_Wb, 2 * NN_NPARAM) == hipSuccess &&
                hipMalloc(&c->nn_Wtb, 2 * NN_NPARAM) == hipSuccess &&
                hipMalloc(&c->nn_cfg, sizeof(NNDev)) == hipSuccess &&
-               hipMalloc(&c->nn_fb, sizeof(int32_t) * ncell) == hipSuccess &&
-               hipMalloc(&c->nn_fbcount, sizeof(int32_t)) == hipSuccess;
+               hipMalloc(&c->nn_status, ncell) == hipSuccess &&
+               hipMalloc(&c->nn_fbcount, sizeof(int32_t) * ((ncell + 63) / 64)) == hipSuccess &&
+               hipMalloc(&c->nn_fblist, sizeof(int32_t) * ncell) == hipSuccess &&
+               hipMalloc(&c->nn_fbtotal, sizeof(int32_t)) == hipSuccess;
     if (!okm) {
       nn_free(c);
       set_error("f3d_nn_init: device allocation failed");
@@ -794,22 +857,29 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
     return F3D_ERR_ARG;
   }
   const long ncell = (long)c->g.Nx * c->g.Ny * c->g.Nz;
-  rc = hip_check(hipMemsetAsync(c->nn_fbcount, 0, sizeof(int32_t), s), "f3d_nn_chem");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_nn_chem, dim3((unsigned)((ncell + 63) / 64)), dim3(64), 0, s, U, Q, rhoi,
-                     tag, c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb,
-                     c->nn_master, c->g, dt, c->nn_fb, c->nn_fbcount, status);
+  const long ntile = (ncell + 63) / 64;
+  int8_t* st = status ? status : c->nn_status;
+  // every tile writes its count, so no clearing is needed
+  hipLaunchKernelGGL(k_nn_chem, dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
+                     c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
+                     c->g, dt, c->nn_fbcount, st);
+  hipLaunchKernelGGL(k_nn_scan, dim3(1), dim3(1024), 0, s, c->nn_fbcount, ntile, c->nn_fbtotal);
+  hipLaunchKernelGGL(k_nn_compact, dim3((unsigned)ntile), dim3(64), 0, s, st, c->nn_fbcount,
+                     ncell, c->nn_fblist);
   rc = hip_check(hipGetLastError(), "f3d_nn_chem");
   if (rc) return rc;
   int32_t nfb = 0;
-  rc = hip_check(hipMemcpyAsync(&nfb, c->nn_fbcount, sizeof(int32_t), hipMemcpyDeviceToHost, s),
+  rc = hip_check(hipMemcpyAsync(&nfb, c->nn_fbtotal, sizeof(int32_t), hipMemcpyDeviceToHost, s),
                  "f3d_nn_chem");
   if (rc) return rc;
   rc = hip_check(hipStreamSynchronize(s), "f3d_nn_chem");
   if (rc) return rc;
   *n_fallback = nfb;
+  // the rejected cells, in cell order: the stiff ones stay grouped in a few
+  // waves (a wave's cost is its slowest cell) instead of one in every wave
   if (nfb > 0)
-    return launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fb, nfb);
+    return launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fblist,
+                       nfb);
   return F3D_OK;
 }
 

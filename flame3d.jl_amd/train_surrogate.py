@@ -1,11 +1,13 @@
 """Train the H2/O2 chemistry surrogate on the GPU and save it.
 
     python flame3d.jl_amd/train_surrogate.py --out PATH [--samples-shape 64 64 32]
-        [--steps 2000] [--records 32] [--epochs 600] [--batch 8192] [--lr 2e-3]
+        [--steps 2000] [--records 32] [--epochs 1500] [--batch 8192] [--lr 2e-3]
 
 Data: trajectories of the library's FP64 backward-Euler Newton integrator
 (f3d_chem_newton, dt = 1e-7 s) from H2/O2/N2 reactors with T ~ U[900, 2500] K,
-P ~ U[1, 10] atm, mixture fraction ~ Beta(2, 2) (NN/H2/gen_data.py pattern).
+P ~ U[1, 10] atm, mixture fraction ~ Beta(2, 2) plus pure streams
+(NN/H2/gen_data.py pattern), and the integrator's steps inside the reacting
+H2/air mixing layer (another seed and resolution than the benchmark's).
 Training: the library's BF16 MFMA forward/backward and Adam kernels
 (f3d_nn_train_step).  Then a held-out check on fresh reactors with a
 different seed: surrogate step (with gates and Newton fallback) against the
@@ -73,14 +75,22 @@ def main(argv=None):
     ap.add_argument("--samples-shape", type=int, nargs=3, default=(64, 64, 32))
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--records", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--epochs", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--decay-every", type=int, default=200)
+    ap.add_argument("--decay-every", type=int, default=500)
     ap.add_argument("--res-tol", type=float, default=3e-5)
     ap.add_argument("--res-T-tol", type=float, default=0.25)
     ap.add_argument("--dT-max", type=float, default=20.0)
+    ap.add_argument("--chi-max", type=float, default=0.05)
+    ap.add_argument("--label-scale", type=float, default=1e-6)
+    ap.add_argument("--elem-tol", type=float, default=1e30)
     ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--flow-steps", type=int, default=8,
+                    help="time steps of the reacting mixing layer sampled (0: reactors only)")
+    ap.add_argument("--flow-size", type=int, default=192)
+    ap.add_argument("--flow-per-call", type=int, default=262144)
+    ap.add_argument("--flow-seed", type=int, default=999)
     a = ap.parse_args(argv)
 
     pkg = _load_package()
@@ -97,10 +107,16 @@ def main(argv=None):
     t0 = time.time()
     log(f"generating data: {a.samples_shape} reactors x {a.records} records over {a.steps} steps")
     data = sur.generate(mech, tuple(a.samples_shape), a.steps, a.records, a.seed, log=log)
+    if a.flow_steps > 0:
+        log(f"generating flow data: {a.flow_size}^3 mixing layer, {a.flow_steps} steps")
+        data = sur.concat(data, sur.generate_flow(mech, (a.flow_size,) * 3, a.flow_steps,
+                                                  a.flow_per_call, a.flow_seed, log=log))
     t_gen = time.time() - t0
     M = data.X.shape[0]
     log(f"  {M} samples in {t_gen:.1f}s, mean Newton iterations {data.newton_iters:.2f}")
-    cfg = sur.make_config(mech, data, res_tol=a.res_tol, res_T_tol=a.res_T_tol, dT_max=a.dT_max)
+    cfg = sur.make_config(mech, data, res_tol=a.res_tol, res_T_tol=a.res_T_tol, dT_max=a.dT_max,
+                           chi_max=a.chi_max, label_scale=a.label_scale,
+                           elem_tol=a.elem_tol)
     Xn, Yn = sur.normalize(cfg, data)
     del data
     # hold out 1/16 for validation

@@ -188,7 +188,8 @@ typedef struct {
   float mahal_mean[11];
   float mahal_inv[121];  /* inverse covariance of the normalized inputs, row-major */
   float d_max;           /* cells with Mahalanobis distance > d_max fall back */
-  float elem_tol;        /* |element balance of dY| / element content > elem_tol falls back */
+  float elem_tol;        /* max_e |net change of element e| / gross elemental turnover
+                            sum_e sum_k a_ek |dY_k| / W_k > elem_tol falls back */
   float neg_tol;         /* Y_k + dY_k < -neg_tol falls back */
   float res_tol;         /* backward-Euler residual max_k |R_k|/rho > res_tol falls back
                             (R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T')); <= 0: off */
@@ -196,6 +197,10 @@ typedef struct {
                             temperature error those residuals imply; <= 0: off */
   float dT_max;          /* predicted chemical temperature change sum_k |h_f,k dY_k| / c_v
                             above dT_max [K] falls back (fast chemistry); <= 0: off */
+  float chi_max;         /* thermal explosiveness at the predicted state,
+                            chi = dt (d HRR / dT) / (rho c_v), HRR = -sum_k h_f,k W_k wdot_k,
+                            above chi_max falls back (where the backward-Euler step can
+                            jump between roots); <= 0: off */
   float boxcox_lambda;
   int32_t label_mode;    /* outputs: 0 dY_k; 1 d BoxCox(Y_k) (post_predict form,
                             Y' = (lambda (BoxCox(Y) + out) + 1)^(1/lambda));
@@ -230,7 +235,7 @@ int f3d_nn_forward(f3d_ctx* ctx, const float* X, float* Y, int32_t B, hipStream_
  * status [Nx*Ny*Nz] (device, may be NULL): 0 untouched, 1 surrogate; Newton
  * fallback after the gate that rejected the cell: 2 Mahalanobis distance,
  * 3 element balance, 4 positivity, 5 predicted temperature change,
- * 6 backward-Euler residual.
+ * 6 backward-Euler residual, 7 thermal explosiveness.
  * *n_fallback (host) receives the number of fallback cells; this call
  * synchronizes the stream to size the fallback launch. */
 int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int32_t maxiter,
