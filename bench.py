@@ -80,7 +80,7 @@ def kernel_bytes(name, R, Ns, npts, nf, nc):
 # ---------------------------------------------------------------------------
 # HBM traffic from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
 # ---------------------------------------------------------------------------
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_mixing_layer_256.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_reacting_mixing_layer_256.json")
 
 
 def _pmc_avg(path):
@@ -117,24 +117,35 @@ def pmc_summary(d):
     return out
 
 
-def traffic_for(kernel_symbol):
-    if not os.path.exists(TRAFFIC_FILE):
+def traffic_for(symbols):
+    """HBM bytes per launch (read + write), launch-weighted over `symbols`."""
+    if not symbols or not os.path.exists(TRAFFIC_FILE):
         return None
-    t = json.load(open(TRAFFIC_FILE))["kernels"].get(kernel_symbol)
-    return None if t is None else t["fetch_bytes"] + t["write_bytes"]
+    ks = json.load(open(TRAFFIC_FILE))["kernels"]
+    tot = 0.0
+    for sym, w in symbols:
+        t = ks.get(sym)
+        if t is None:
+            return None
+        tot += w * (t["fetch_bytes"] + t["write_bytes"])
+    return tot
 
 
-def kernel_symbol(name, Ns):
-    """Kernel symbol (as rocprofv3 prints it) of each timed kernel on the FP32 path."""
+def kernel_symbols(name, Ns):
+    """(rocprofv3 kernel symbol, share of the launches) of each timed kernel on
+    the FP32 path: x sweeps use the per-face kernels, y and z sweeps the
+    marching ones.  None where one symbol serves launches of different sizes
+    (the WENO kernels run for 5 and for Ns variables)."""
     return {
-        "flux_split": "void f3d::k_flux_split<float>",
-        "species_split": "void f3d::k_species_split<float>",
-        "weno5": "void f3d::k_weno<float>", "weno_species": "void f3d::k_weno<float>",
-        "viscous": "void f3d::k_viscous<float>",
-        "spec_viscous": f"void f3d::k_spec_viscous<{Ns}, float>",
-        "shock_sensor": "void f3d::k_shock_sensor<float>",
-        "mixture": f"void f3d::k_mixture<{Ns}, float>",
-    }[name]
+        "flux_split": [("void f3d::k_flux_split<float, float>", 1.0)],
+        "species_split": [("void f3d::k_species_split<float, float>", 1.0)],
+        "viscous": [("void f3d::k_viscous<float, float>", 1 / 3),
+                    ("void f3d::k_viscous_march<float>", 2 / 3)],
+        "spec_viscous": [(f"void f3d::k_spec_viscous<{Ns}, float, float>", 1 / 3),
+                         (f"void f3d::k_spec_viscous_march<{Ns}, float>", 2 / 3)],
+        "shock_sensor": [("void f3d::k_shock_sensor<float>", 1.0)],
+        "mixture": [(f"void f3d::k_mixture<{Ns}, float>", 1.0)],
+    }.get(name)
 
 
 def time_kernels(ctx, torch, Q, rhoi, metrics, J, reps=10):
@@ -223,7 +234,8 @@ def strang_step(ctx, st, dt, chem):
     ctx.fill(Q, U, rhoi)
 
 
-def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate=None):
+def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate=None,
+             chemistry=True):
     """Time `steps` full time steps of the path.  prec fp64: FP64 stage +
     FP64 backward-Euler Newton chemistry (the plain method); fp32 with a
     surrogate: FP32 flux arithmetic + the BF16 MLP chemistry step with Newton
@@ -239,7 +251,10 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
     ctx.set_boundary(case.bc)
     st = (U, Un, rhoi, rhon, Q)
     chem_ms, fallback = [], []
-    if surrogate is not None:
+    if not chemistry:
+        def chem(h):
+            pass
+    elif surrogate is not None:
         surrogate.attach(ctx)
 
         def chem(h):
@@ -275,7 +290,7 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
-    cms = [a.elapsed_time(b) for a, b in chem_ms]
+    cms = [a.elapsed_time(b) for a, b in chem_ms] or [0.0]
     kt = time_kernels(ctx, torch, Q, rhoi, case.grid.metrics, case.grid.J, kernel_reps)
     return ctx, dict(ms_total=ms, wall_s=wall, Q=Q, rhoi=rhoi, U=U, kernels=kt,
                      chem_ms_mean=sum(cms) / len(cms), chem_calls=len(cms),
@@ -339,6 +354,42 @@ def cpu_baseline(mech, size, steps, min_seconds=10.0):
                       f"Strang Newton chemistry), {dt:.1f} s"}
 
 
+def bench_air(a, torch, abi, cases, mm):
+    """The shock-laden mixing layer of 5-species air, non-reacting: one RK3
+    stage (the step) on the FP32 path against the FP64 path."""
+    mech = mm.load("air5")
+    N = (a.size, a.size, a.size)
+    case = cases.mixing_layer(N, mech, dev="cuda")
+    nc = N[0] * N[1] * N[2]
+    stages = 3 * a.steps
+    c64, r64 = run_path("fp64", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        chemistry=False)
+    c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        chemistry=False)
+    ms64, ms32 = r64["ms_total"] / stages, r32["ms_total"] / stages
+    errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
+    npts = (N[0] + 8) * (N[1] + 8) * (N[2] + 8)
+    nf = (N[0] + 1) * N[1] * N[2]
+    ktab = {}
+    for prec, res in (("fp32", r32), ("fp64", r64)):
+        R = 4 if prec == "fp32" else 8
+        for name, (ms, per) in res["kernels"].items():
+            b = kernel_bytes(name, R, mech.Ns, npts, nf, nc)
+            ktab.setdefault(prec, {})[name] = {
+                "ms": ms, "launches_per_stage": per, "alg_bytes": b,
+                "GBps": b / (ms * 1e-3) / 1e9, "frac_hbm": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    print(json.dumps({
+        "metric": "cell-updates/s per RK stage, FP32 flux vs FP64 (non-reacting air)",
+        "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": stages,
+        "warmup": a.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 flux, f64 state", "data": "synthetic",
+        "config": {"workload": f"shock-laden mixing layer {N[0]}^3, 5-species air, non-reacting"},
+        "fp64": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64},
+        "speedup_fp32_vs_fp64": ms64 / ms32, "error_fp32_vs_fp64": errs, "kernels": ktab}))
+    c32.close()
+    c64.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,8 +402,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--workload", choices=("reacting", "air"), default="reacting",
+                    help="air: the non-reacting 5-species air mixing layer, FP32 vs FP64 stage")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="write the per-kernel HBM traffic summary from rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-out", metavar="FILE", help="where --pmc-summary writes (default: "
+                    "profiles/hbm_traffic_reacting_mixing_layer_256.json)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU's block only; --gpus {a.gpus} is not "
@@ -362,8 +417,9 @@ def main():
 
     if a.pmc_summary:
         res = pmc_summary(a.pmc_summary)
-        os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
-        with open(TRAFFIC_FILE, "w") as fh:
+        out = a.pmc_out or TRAFFIC_FILE
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        with open(out, "w") as fh:
             json.dump(res, fh, indent=1)
         print(json.dumps(res, indent=1))
         return
@@ -377,6 +433,8 @@ def main():
     import flame3d_amd.mech as mm
     import flame3d_amd.surrogate as sur
 
+    if a.workload == "air":
+        return bench_air(a, torch, abi, cases, mm)
     mech = mm.load("h2_lidryer")
     N = (a.size, a.size, a.size)
     log(f"[bench] building the reacting mixing-layer case {N} on the GPU")
@@ -420,7 +478,7 @@ def main():
     dk = ktab["fp32"][dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": dk["GBps"] / HBM_PEAK_GBS,
-                "traffic": traffic_for(kernel_symbol(dom, Ns)),
+                "traffic": traffic_for(kernel_symbols(dom, Ns)),
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["ms"]}
 
     cpu = None
