@@ -31,6 +31,7 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
   double rs[NS], sc[NS], wdot[NS];
 #pragma unroll
   for (int n = 0; n < NS; ++n) rs[n] = r[n];
+  double dT_prev = 0.0;
   for (int k = 0; k < maxiter; ++k) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) sc[n] = r[n] / P->mw[n] * 1e-6;
@@ -86,8 +87,13 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
     }
     T += dx[NS];
     ++it;
+    // NASA-7 polynomials jump at T_mid: when E0 falls in such a jump the T
+    // update flips sign without contracting; with the species converged the
+    // step is accepted (the final T comes from GetT of the energy)
+    const bool flip = k > 0 && dx[NS] * dT_prev < 0 && fabs(dx[NS]) >= 0.5 * fabs(dT_prev);
+    dT_prev = dx[NS];
     if (bad || !(T > T_lo && T < T_hi)) return false;
-    if (mx <= rtol * rho && fabs(dx[NS]) <= ttol) return true;
+    if (mx <= rtol * rho && (fabs(dx[NS]) <= ttol || flip)) return true;
   }
   return false;
 }

@@ -1153,6 +1153,7 @@ static bool be_solve(const Grid& g, const or_dims* d, const or_thermo* th,
   double rs[64], s[64], wdot[64], wdot2[64], eis[64], rhs[64], dx[64];
   std::vector<double> Arate(Ns * Ns), A(N1 * N1);
   for (int n = 0; n < Ns; ++n) rs[n] = r[n];
+  double dT_prev = 0.0;
   for (int k = 0; k < maxiter; ++k) {
     for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
     or_production_rate(d, th, kin, s, T, kEps, 1, wdot, Arate.data());
@@ -1183,8 +1184,13 @@ static bool be_solve(const Grid& g, const or_dims* d, const or_thermo* th,
     }
     T += dx[Ns];
     ++it;
+    // NASA-7 polynomials jump at T_mid: when E0 falls in such a jump the T
+    // update flips sign without contracting; with the species converged the
+    // step is accepted (the final T comes from GetT of the energy)
+    const bool flip = k > 0 && dx[Ns] * dT_prev < 0 && std::fabs(dx[Ns]) >= 0.5 * std::fabs(dT_prev);
+    dT_prev = dx[Ns];
     if (bad || !(T > T_lo && T < T_hi)) return false;
-    if (mx <= rtol * rho && std::fabs(dx[Ns]) <= ttol) return true;
+    if (mx <= rtol * rho && (std::fabs(dx[Ns]) <= ttol || flip)) return true;
   }
   return false;
 }

@@ -123,3 +123,39 @@ def test_chem_newton_restart_mid_ignition():
     assert np.array_equal(it_o, it_g), np.argwhere(it_o != it_g)[:10]
     _compare((Qo, ro, Uo, None), (to_np(Qg), to_np(rg), to_np(Ug), None))
     ctx.close()
+
+
+def test_chem_newton_nasa7_breakpoint_gap():
+    """The states of tests/golden/newton_nasa_gap.json (energy inside the
+    NASA-7 jump at 1000 K) on the GPU: same iteration counts as the oracle,
+    fields to 1e-12, no failed cell."""
+    import json
+    import os
+    import flame3d_amd.abi as abi
+    import flame3d_amd.mech as mm
+    from oracle import Oracle
+    print("\n[chem] newton at the NASA-7 breakpoint", flush=True)
+    path = os.path.join(os.path.dirname(__file__), "golden", "newton_nasa_gap.json")
+    cells = [[float.fromhex(x) for x in row] for row in json.load(open(path))["cells"]]
+    m = mm.load("h2_lidryer")
+    size = (4, 4, 4)  # the three states repeated over the block
+    Q = np.zeros((12, 12, 12, 7), order="F")
+    R = np.zeros((12, 12, 12, 9), order="F")
+    U = np.zeros((12, 12, 12, 5), order="F")
+    for i in range(4):
+        for j in range(4):
+            for k in range(4):
+                row = cells[(i + j + k) % len(cells)]
+                Q[4 + i, 4 + j, 4 + k] = row[:7]
+                R[4 + i, 4 + j, 4 + k] = row[7:16]
+                U[4 + i, 4 + j, 4 + k] = row[16:21]
+    Qg, Rg, Ug = to_torch(Q), to_torch(R), to_torch(U)
+    it_o = Oracle(*size, m).chem_newton(U, Q, R, 1e-7)
+    ctx = abi.Context(*size, m, "fp64")
+    it_g = torch.zeros((4, 4, 4), dtype=torch.int32, device="cuda")
+    ctx.chem_newton(Ug, Qg, Rg, 1e-7, iters=it_g)
+    torch.cuda.synchronize()
+    it_g = it_g.cpu().numpy().T
+    assert it_o.min() >= 1 and np.array_equal(it_o, it_g), (it_o.ravel(), it_g.ravel())
+    _compare((Q, R, U, None), (to_np(Qg), to_np(Rg), to_np(Ug), None))
+    ctx.close()

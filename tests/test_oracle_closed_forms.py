@@ -522,3 +522,24 @@ def test_bug_compat_flag_reads_transposed_coefficients(h2):
         orc.chem_stiff(Uo, Qo, ro, 1e-7, bug_compat=bug)
         outs.append(Uo[..., 4].copy())
     assert not np.allclose(outs[0], outs[1])
+
+
+def test_newton_accepts_nasa7_breakpoint_gap(h2):
+    """Regression: states whose energy falls in the jump of the NASA-7
+    polynomials at T_mid (fixture tests/golden/newton_nasa_gap.json) made the
+    T update flip sign forever; the step is accepted once the species have
+    converged (2 iterations, not a failure after every substep restart)."""
+    import json
+    import os
+    from oracle import Oracle
+    path = os.path.join(os.path.dirname(__file__), "golden", "newton_nasa_gap.json")
+    cells = [[float.fromhex(x) for x in row] for row in json.load(open(path))["cells"]]
+    orc = Oracle(1, 1, 1, h2)
+    for row in cells:
+        Q = np.zeros((9, 9, 9, 7), order="F")
+        R = np.zeros((9, 9, 9, 9), order="F")
+        U = np.zeros((9, 9, 9, 5), order="F")
+        Q[:], R[:], U[:] = row[:7], row[7:16], row[16:21]
+        it = orc.chem_newton(U, Q, R, 1e-7)
+        assert 1 <= int(it.ravel()[0]) <= 4, it.ravel()[0]
+        assert abs(Q[4, 4, 4, 5] - 1000.0) < 1.0  # T from GetT stays at the stream's 1000 K
