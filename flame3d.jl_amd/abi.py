@@ -63,9 +63,10 @@ class NNConfig(C.Structure):
                 ("mahal_mean", C.c_float * 11), ("mahal_inv", C.c_float * 121),
                 ("d_max", C.c_float), ("elem_tol", C.c_float), ("neg_tol", C.c_float),
                 ("res_tol", C.c_float), ("res_T_tol", C.c_float), ("dT_max", C.c_float),
-                ("chi_max", C.c_float),
+                ("chi_max", C.c_float), ("chain_max", C.c_float), ("chain_species", C.c_int32),
                 ("boxcox_lambda", C.c_float), ("label_mode", C.c_int32),
                 ("label_scale", C.c_float * 9),
+                ("label_rel", C.c_float), ("label_floor", C.c_float), ("y_floor", C.c_float),
                 ("n_elem", C.c_int32),
                 ("elem", (C.c_float * 9) * 4), ("dt", C.c_double)]
 

@@ -59,9 +59,14 @@ struct NNDev {
   float res_T_tol;                 // temperature-weighted residual gate [K] (<= 0: off)
   float dT_max;                    // predicted chemical temperature change gate [K] (<= 0: off)
   float chi_max;                   // thermal explosiveness gate (<= 0: off)
+  float chain_max;                 // chain explosiveness gate (<= 0: off)
+  int chain_idx[3];                // chain-carrier species of that gate (-1: unused)
   float boxcox_lambda;             // Y -> (Y^lambda - 1)/lambda input transform
-  int label_mode;                  // 0: dY; 1: d BoxCox(Y); 2: asinh(dY / label_scale)
+  int label_mode;                  // 0: dY; 1: d BoxCox(Y); 2: asinh(dY / label_scale);
+                                   // 3: asinh(dY / s(Y)), s = min(scale, max(floor, rel Y))
   float label_scale[NN_OUT];
+  float label_rel, label_floor;    // label_mode 3
+  float y_floor;                   // inputs use BoxCox(max(Y, y_floor))
   int n_elem;
   float elem[4][NN_OUT];           // atoms of element e per molecule of species k
   double dt;                       // chemistry time step the network was trained for

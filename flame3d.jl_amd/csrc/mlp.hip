@@ -347,11 +347,93 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
   for (int o = 0; o < NN_OUT; ++o) Yg[(s0 + lane) * NN_OUT + o] = L.Y[lane][o];
 }
 
+// Element-conserving projection with per-cell weights S:
+// dY = dY0 - S E^T (E S E^T)^-1 E dY0, E[e][k] = atoms_ek / W_k, solved as a
+// 4x4 system (rows past n_elem are identity) by Gaussian elimination with
+// partial pivoting in registers (compile-time indices, row swaps by selects).
+// Returns false if E S E^T is singular (dY = dY0 then).
+template <int NS>
+__device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
+                                                 const DevParams* __restrict__ P,
+                                                 const double (&S)[NS], const double (&dY0)[NS],
+                                                 double (&dY)[NS]) {
+  const int ne = C->n_elem;
+  double G[4][4], b[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    b[a] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) G[a][c] = (a == c && a >= ne) ? 1.0 : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    double e[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) e[a] = a < ne ? (double)C->elem[a][k] / P->mw[k] : 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      b[a] += e[a] * dY0[k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) G[a][c] += e[a] * S[k] * e[c];
+    }
+  }
+  bool ok = true;
+#pragma unroll
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    double pv = fabs(G[col][col]);
+#pragma unroll
+    for (int a = col + 1; a < 4; ++a)
+      if (fabs(G[a][col]) > pv) {
+        pv = fabs(G[a][col]);
+        piv = a;
+      }
+    if (!(pv > 0.0)) ok = false;
+#pragma unroll
+    for (int a = col + 1; a < 4; ++a) {
+      const bool sw = (piv == a);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double t = G[col][c];
+        G[col][c] = sw ? G[a][c] : t;
+        G[a][c] = sw ? t : G[a][c];
+      }
+      const double t = b[col];
+      b[col] = sw ? b[a] : t;
+      b[a] = sw ? t : b[a];
+    }
+    const double inv = ok ? 1.0 / G[col][col] : 0.0;
+#pragma unroll
+    for (int a = col + 1; a < 4; ++a) {
+      const double f = G[a][col] * inv;
+#pragma unroll
+      for (int c = col; c < 4; ++c) G[a][c] -= f * G[col][c];
+      b[a] -= f * b[col];
+    }
+  }
+  double z[4];
+#pragma unroll
+  for (int a = 3; a >= 0; --a) {
+    double v = b[a];
+#pragma unroll
+    for (int c = a + 1; c < 4; ++c) v -= G[a][c] * z[c];
+    z[a] = ok ? v / G[a][a] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) v += (a < ne ? (double)C->elem[a][k] / P->mw[k] : 0.0) * z[a];
+    dY[k] = dY0[k] - S[k] * v;
+  }
+  return ok;
+}
+
 // ---------------------------------------------------------------------------
 // Fused surrogate chemistry step (pre_input -> network -> post_predict,
 // reactions.jl:2-56): one wave = 64 interior cells.  The network runs for all
 // 64 lanes (uniform control flow around the MFMAs); each active lane then
-// applies six gates and either commits the prediction, with the FP64 energy
+// applies seven gates and either commits the prediction, with the FP64 energy
 // bookkeeping of the backward-Euler integrator (chem_epilogue), or marks its
 // cell for the Newton pass and leaves it untouched (status = 1 + gate); the
 // marked cells are then listed in cell order (k_nn_scan, k_nn_compact):
@@ -366,7 +448,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
 //      R_k = rho_k' - rho_k - dt W_k wdot_k(rho', T'), T' from the energy (error gate);
 //   6. thermal explosiveness chi = dt (d HRR / dT) / (rho c_v) <= chi_max at the
 //      predicted state: near chi ~ 1 the backward-Euler equation can have a
-//      second (ignited) root that a small residual does not rule out.
+//      second (ignited) root that a small residual does not rule out;
+//   7. chain explosiveness: every eigenvalue of dt d wdot / d c over the chain
+//      carriers below chain_max (near dt lambda = 1 a hot radical-free mixture
+//      has a near-root with a tiny residual while the Newton step ignites it).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_nn_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
@@ -407,7 +492,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       r[n] = rhoi[p + n * NP];
-      const float Y = (float)fmax(r[n] * rinv, 0.0);
+      const float Y = (float)fmax(r[n] * rinv, (double)C->y_floor);
       x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
     }
 #pragma unroll
@@ -417,7 +502,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? xn[k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
-  int code = act ? 1 : 0;  // 1 accepted, 2..7 the first gate that failed
+  int code = act ? 1 : 0;  // 1 accepted, 2..8 the first gate that failed
   if (act) {
     // 1. Mahalanobis distance of the normalized inputs
     float d2 = 0.0f;
@@ -432,7 +517,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     // post_predict (reactions.jl:26-56) on the de-normalized outputs
     const double rinv = 1.0 / fmax(rho, kEps);
     const double lam = (double)C->boxcox_lambda;
-    double Y[NS], dY0[NS], dY[NS];
+    double Y[NS], dY0[NS], dY[NS], S[NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       Y[n] = fmax(r[n] * rinv, 0.0);
@@ -441,6 +526,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         dY0[n] = o;
       } else if (C->label_mode == 2) {
         dY0[n] = (double)C->label_scale[n] * sinh(o);
+      } else if (C->label_mode == 3) {
+        const double sk = fmin((double)C->label_scale[n],
+                               fmax((double)C->label_floor, (double)C->label_rel * Y[n]));
+        dY0[n] = sk * sinh(o);
+        S[n] = (sk + fabs(dY0[n])) * (sk + fabs(dY0[n])) + 1e-60;
       } else {
         const double base = lam * ((pow(Y[n], lam) - 1.0) / lam + o) + 1.0;
         dY0[n] = (base > 0.0 ? pow(base, 1.0 / lam) : 0.0) - Y[n];
@@ -465,17 +555,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       }
       if (gross > 0.0 && worst > (double)C->elem_tol * gross) eok = false;
     }
+    if (C->label_mode == 3) {
+      // per-cell weights S_k = (s_k + |dY0_k|)^2: dY = dY0 - S E^T (E S E^T)^-1 E dY0
+      if (!project_weighted<NS>(C, P, S, dY0, dY)) eok = false;
+    } else {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) v += C->proj[n][m] * dY0[m];
+        dY[n] = v;
+      }
+    }
     if (code == 1 && !eok) code = 3;
     bool pok = true;
 #pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      double v = 0.0;
-#pragma unroll
-      for (int m = 0; m < NS; ++m) v += C->proj[n][m] * dY0[m];
-      dY[n] = v;
-      // 3. positivity
+    for (int n = 0; n < NS; ++n)  // 3. positivity
       if (Y[n] + dY[n] < -(double)C->neg_tol) pok = false;
-    }
     if (code == 1 && !pok) code = 4;
     // 4. fast chemistry: the temperature change the prediction implies
     if (code == 1 && C->dT_max > 0.0f) {
@@ -502,7 +598,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       // 5. backward-Euler residual at (rho', T'): its largest species entry,
       //    and the temperature error its formation energies imply
       const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;
-      if (res_on || C->chi_max > 0.0f) {
+      if (res_on || C->chi_max > 0.0f || C->chain_max > 0.0f) {
         double sc[NS], wdot[NS], Adum[NS][NS];
 #pragma unroll
         for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
@@ -530,6 +626,42 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
             dq -= P->lo[n][5] * P->Ru * 1e6 * (w2[n] - wdot[n]);  // h_f,k W_k d(wdot_k)
           if (!(dt * dq / hT <= (double)C->chi_max * cvn)) code = 7;
         }
+        // 7. chain explosiveness: the 3x3 block M = dt d wdot / d c over the
+        //    chain carriers (forward differences, eps = 1e-8 of the total
+        //    concentration), rejected unless every eigenvalue of M has real
+        //    part < chain_max: Routh-Hurwitz on N = chain_max I - M
+        //    (tr N > 0, det N > 0, tr N * m2(N) > det N, m2 = principal minors)
+        if (code == 1 && C->chain_max > 0.0f) {
+          double ct = 0.0, Nm[3][3];
+#pragma unroll
+          for (int n = 0; n < NS; ++n) ct += sc[n];
+          const double eps = 1e-8 * ct;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const int cj = C->chain_idx[j];
+            double sc2[NS], w3[NS];
+#pragma unroll
+            for (int n = 0; n < NS; ++n) sc2[n] = sc[n] + (n == cj ? eps : 0.0);
+            if (cj >= 0) production_rate<NS, false>(P, sc2, Tn, Adum, w3);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const int ci = C->chain_idx[i];
+              double d = 0.0;
+#pragma unroll
+              for (int n = 0; n < NS; ++n) d += (n == ci) ? (w3[n] - wdot[n]) : 0.0;
+              const double Mij = (cj >= 0 && ci >= 0) ? dt * d / eps : 0.0;
+              Nm[i][j] = (i == j ? (double)C->chain_max : 0.0) - Mij;
+            }
+          }
+          const double tr = Nm[0][0] + Nm[1][1] + Nm[2][2];
+          const double m2 = Nm[0][0] * Nm[1][1] - Nm[0][1] * Nm[1][0] +
+                            Nm[0][0] * Nm[2][2] - Nm[0][2] * Nm[2][0] +
+                            Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1];
+          const double det = Nm[0][0] * (Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1]) -
+                             Nm[0][1] * (Nm[1][0] * Nm[2][2] - Nm[1][2] * Nm[2][0]) +
+                             Nm[0][2] * (Nm[1][0] * Nm[2][1] - Nm[1][1] * Nm[2][0]);
+          if (!(tr > 0.0 && det > 0.0 && tr * m2 > det)) code = 8;
+        }
       }
       if (code == 1) {
         U[p + 4 * NP] += dei;
@@ -545,7 +677,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   const bool ok = (code == 1);
   // rejected cells of this 64-cell tile (status >= 2); the ordered list the
   // Newton pass runs over is built from these counts (k_nn_scan, k_nn_compact)
-  const bool rej = act && !ok;  // code 2..7
+  const bool rej = act && !ok;  // code 2..8
   const unsigned long long bal = __ballot(rej);
   if (lane == 0) tile_count[tile] = (int32_t)__popcll(bal);
 }
@@ -683,9 +815,18 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
     return F3D_ERR_UNSUPPORTED;
   }
   if (cfg->n_elem < 0 || cfg->n_elem > 4 || !(cfg->dt > 0) || !(cfg->boxcox_lambda > 0) ||
-      cfg->label_mode < 0 || cfg->label_mode > 2) {
-    set_error("f3d_nn_init: need 0 <= n_elem <= 4, dt > 0, boxcox_lambda > 0, label_mode 0..2");
+      cfg->label_mode < 0 || cfg->label_mode > 3 || !(cfg->y_floor >= 0)) {
+    set_error("f3d_nn_init: need 0 <= n_elem <= 4, dt > 0, boxcox_lambda > 0, label_mode 0..3, "
+              "y_floor >= 0");
     return F3D_ERR_ARG;
+  }
+  if (cfg->label_mode == 3) {
+    bool okl = cfg->label_rel > 0 && cfg->label_floor > 0;
+    for (int k = 0; k < NN_OUT; ++k) okl = okl && cfg->label_scale[k] >= cfg->label_floor;
+    if (!okl) {
+      set_error("f3d_nn_init: label_mode 3 needs label_rel > 0 and label_scale >= label_floor > 0");
+      return F3D_ERR_ARG;
+    }
   }
   for (int k = 0; k < NN_IN; ++k)
     if (!(cfg->in_std[k] > 0)) {
@@ -711,9 +852,29 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   h.res_T_tol = cfg->res_T_tol;
   h.dT_max = cfg->dT_max;
   h.chi_max = cfg->chi_max;
+  h.chain_max = cfg->chain_max;
+  {
+    int nc = 0;
+    for (int j = 0; j < 3; ++j) h.chain_idx[j] = -1;
+    for (int k = 0; k < NN_OUT; ++k)
+      if ((cfg->chain_species >> k) & 1) {
+        if (nc == 3) {
+          set_error("f3d_nn_init: chain_species names more than 3 species");
+          return F3D_ERR_ARG;
+        }
+        h.chain_idx[nc++] = k;
+      }
+    if ((cfg->chain_species >> NN_OUT) != 0 || (cfg->chain_max > 0 && nc == 0)) {
+      set_error("f3d_nn_init: chain_species must name 1..3 of the 9 species when chain_max > 0");
+      return F3D_ERR_ARG;
+    }
+  }
   h.boxcox_lambda = cfg->boxcox_lambda;
   h.label_mode = cfg->label_mode;
   for (int k = 0; k < NN_OUT; ++k) h.label_scale[k] = cfg->label_scale[k];
+  h.label_rel = cfg->label_rel;
+  h.label_floor = cfg->label_floor;
+  h.y_floor = cfg->y_floor;
   h.n_elem = cfg->n_elem;
   for (int e = 0; e < 4; ++e)
     for (int k = 0; k < NN_OUT; ++k) h.elem[e][k] = cfg->elem[e][k];

@@ -104,14 +104,18 @@ class Dataset:
     Y1: "object"  # (M, 9) FP64 mass fractions after one step of DT, device
     newton_iters: float = 0.0  # mean Newton iterations per sample
 
-    def labels(self, label_mode: int, label_scale=None):
+    def labels(self, label_mode: int, label_scale=None, label_rel: float = 0.0,
+               label_floor: float = 0.0):
         """0: dY; 1: d BoxCox(Y) (the post_predict form, reactions.jl:26-56);
-        2: asinh(dY / label_scale), logarithmic in |dY| above the scale."""
+        2: asinh(dY / label_scale), logarithmic in |dY| above the scale;
+        3: asinh(dY / s(Y0)), s = min(label_scale, max(label_floor, label_rel Y0))."""
         import torch
         if label_mode == 0:
             return self.Y1 - self.Y0
-        if label_mode == 2:
+        if label_mode in (2, 3):
             sc = torch.as_tensor(label_scale, dtype=self.Y0.dtype, device=self.Y0.device)
+            if label_mode == 3:
+                sc = torch.minimum(sc, (label_rel * self.Y0).clamp_min(label_floor))
             return torch.asinh((self.Y1 - self.Y0) / sc)
         return boxcox(self.Y1) - boxcox(self.Y0)
 
@@ -119,19 +123,33 @@ class Dataset:
 def generate(mech, shape=(64, 64, 32), steps: int = 2000, n_record: int = 32, seed: int = SEED,
              maxiter: int = 50, rtol: float = 1e-10, ttol: float = 1e-6,
              T_range=(900.0, 2500.0), P_atm=(1.0, 10.0), z_pure: float = 0.1,
-             log=print) -> Dataset:
+             trace: float = 0.0, log=print) -> Dataset:
     """Trajectory samples (NN/H2/gen_data.py pattern): Nx*Ny*Nz reactors drawn
     from the H2/O2/N2 distribution (T ~ U[900, 2500] K, P ~ U[1, 10] atm,
     mixture fraction ~ Beta(2, 2), plus a fraction z_pure each of pure
     oxidizer and pure fuel, the free streams of a non-premixed flow) are
     advanced by `steps` backward-Euler steps of DT on the GPU; at `n_record`
     steps (the first four, then random) every reactor contributes the pair
-    (state, state after one step)."""
+    (state, state after one step).  With trace > 0 every species of every
+    reactor is seeded with a trace, log-uniform in [1e-14, trace] (the states
+    that diffusion leaves in the free streams of a flow)."""
     import torch
     Nx, Ny, Nz = shape
     ctx = abi.Context(Nx, Ny, Nz, mech, precision="fp64")
     Q, rhoi = cases.chem_cells(shape, mech, seed=seed, dev="cuda", T_range=T_range, P_atm=P_atm,
                                z_pure=z_pure)
+    if trace > 0:
+        g = torch.Generator(device="cuda").manual_seed(seed + 1)
+        rho = Q[0]
+        lo, hi = math.log(1e-14), math.log(trace)
+        tr = torch.exp(lo + (hi - lo) * torch.rand(rhoi.shape, generator=g, dtype=torch.float64,
+                                                   device="cuda"))
+        Y = rhoi / rho + tr
+        Y = Y / Y.sum(0, keepdim=True)
+        T, P = Q[5].clone(), Q[4].clone()
+        zero = torch.zeros_like(T)
+        Q, rhoi = cases.state_from_TPY(mech, T, P, Y, zero, zero, zero)
+        Q, rhoi = Q.contiguous(), rhoi.contiguous()
     U = torch.zeros((5,) + tuple(Q.shape[1:]), dtype=torch.float64, device="cuda")
     rng = np.random.default_rng(seed)
     rec = set(range(min(4, n_record)))
@@ -212,18 +230,36 @@ def _elements(mech):
 
 
 LABEL_SCALE = 1e-6  # asinh label scale: dY below it is linear, above it logarithmic
+LABEL_REL = 0.1     # label_mode 3: below Y = LABEL_SCALE / LABEL_REL the scale follows 0.1 Y
+LABEL_FLOOR = 1e-14  # ... down to this floor (species at zero)
+Y_FLOOR = 1e-12     # inputs see Y below this as this (traces below it do not move the features)
+CHAIN_MAX = 0.5     # chain-explosiveness gate: largest real eigenvalue of dt d wdot / d c
+CHAIN_SPECIES = ("H", "O", "OH")  # ... over the chain carriers
 
 
-def make_config(mech, data: Dataset, label_mode: int = 2, label_scale: float = LABEL_SCALE,
+def floor_inputs(X, y_floor: float, lam: float = LAMBDA):
+    """Raw inputs with BoxCox(Y_k) taken at max(Y_k, y_floor) (BoxCox is
+    increasing, so this is a clamp of the Box-Cox columns)."""
+    if not y_floor > 0:
+        return X
+    import torch
+    lo = (y_floor ** lam - 1.0) / lam
+    return torch.cat([X[:, :2], X[:, 2:].clamp_min(lo)], dim=1)
+
+
+def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = LABEL_SCALE,
                 elem_tol: float = 1e30, neg_tol: float = 1e-6, res_tol: float = 3e-5,
                 res_T_tol: float = 0.25, dT_max: float = 20.0, chi_max: float = 0.05,
-                sigmas: float = 3.0,
+                chain_max: float = CHAIN_MAX, chain_species=CHAIN_SPECIES,
+                label_rel: float = LABEL_REL, label_floor: float = LABEL_FLOOR,
+                y_floor: float = Y_FLOOR, sigmas: float = 3.0,
                 ridge: float = 1e-6) -> abi.NNConfig:
     """Normalization and gate statistics from the training data.  The
     Mahalanobis threshold is the training distances' quantile at the one-sided
     `sigmas`-sigma coverage of a normal law (0.99865 for 3 sigma)."""
     import torch
-    X, lab = data.X, data.labels(label_mode, [label_scale] * NN_OUT)
+    X = floor_inputs(data.X, y_floor)
+    lab = data.labels(label_mode, [label_scale] * NN_OUT, label_rel, label_floor)
     if not (bool(torch.isfinite(X).all()) and bool(torch.isfinite(lab).all())):
         raise ValueError("training data hold non-finite values")
     in_mean = X.mean(0)
@@ -256,10 +292,15 @@ def make_config(mech, data: Dataset, label_mode: int = 2, label_scale: float = L
     cfg.res_T_tol = res_T_tol
     cfg.dT_max = dT_max
     cfg.chi_max = chi_max
+    cfg.chain_max = chain_max
+    cfg.chain_species = sum(1 << mech.index(n) for n in chain_species)
     cfg.boxcox_lambda = LAMBDA
     cfg.label_mode = label_mode
     for k in range(NN_OUT):
         cfg.label_scale[k] = label_scale
+    cfg.label_rel = label_rel
+    cfg.label_floor = label_floor
+    cfg.y_floor = y_floor
     names, comp = _elements(mech)
     cfg.n_elem = len(names)
     for e in range(len(names)):
@@ -277,8 +318,9 @@ def normalize(cfg: abi.NNConfig, data: Dataset):
     isd = torch.tensor(list(cfg.in_std), dtype=torch.float64, device=dev)
     om = torch.tensor(list(cfg.out_mean), dtype=torch.float64, device=dev)
     osd = torch.tensor(list(cfg.out_std), dtype=torch.float64, device=dev)
-    Xn = ((data.X - im) / isd).float().contiguous()
-    Yn = ((data.labels(cfg.label_mode, list(cfg.label_scale)) - om) / osd).float().contiguous()
+    Xn = ((floor_inputs(data.X, cfg.y_floor) - im) / isd).float().contiguous()
+    lab = data.labels(cfg.label_mode, list(cfg.label_scale), cfg.label_rel, cfg.label_floor)
+    Yn = ((lab - om) / osd).float().contiguous()
     return Xn, Yn
 
 
@@ -321,7 +363,9 @@ def mse(ctx: abi.Context, Xn, Yn) -> float:
 _CFG_ARRAYS = ("in_mean", "in_std", "out_mean", "out_std", "mahal_mean", "mahal_inv",
                "label_scale")
 _CFG_SCALARS = ("d_max", "elem_tol", "neg_tol", "res_tol", "res_T_tol", "dT_max", "chi_max",
-                "boxcox_lambda", "label_mode", "n_elem", "dt")
+                "chain_max", "chain_species",
+                "boxcox_lambda", "label_mode", "label_rel", "label_floor", "y_floor", "n_elem",
+                "dt")
 
 
 def save(path: str, cfg: abi.NNConfig, params: np.ndarray, meta: Optional[Dict] = None):

@@ -2,12 +2,14 @@
 
     python flame3d.jl_amd/train_surrogate.py --out PATH [--samples-shape 64 64 32]
         [--steps 2000] [--records 32] [--epochs 1500] [--batch 8192] [--lr 2e-3]
+        [--label-mode 3] [--trace 1e-6] [--flow-steps 8]
 
 Data: trajectories of the library's FP64 backward-Euler Newton integrator
 (f3d_chem_newton, dt = 1e-7 s) from H2/O2/N2 reactors with T ~ U[900, 2500] K,
 P ~ U[1, 10] atm, mixture fraction ~ Beta(2, 2) plus pure streams
-(NN/H2/gen_data.py pattern), and the integrator's steps inside the reacting
-H2/air mixing layer (another seed and resolution than the benchmark's).
+(NN/H2/gen_data.py pattern), the same reactors seeded with species traces,
+and the integrator's steps inside the reacting H2/air mixing layer (another
+seed and resolution than the benchmark's).
 Training: the library's BF16 MFMA forward/backward and Adam kernels
 (f3d_nn_train_step).  Then a held-out check on fresh reactors with a
 different seed: surrogate step (with gates and Newton fallback) against the
@@ -83,7 +85,15 @@ def main(argv=None):
     ap.add_argument("--res-T-tol", type=float, default=0.25)
     ap.add_argument("--dT-max", type=float, default=20.0)
     ap.add_argument("--chi-max", type=float, default=0.05)
+    ap.add_argument("--chain-max", type=float, default=0.5)
+    ap.add_argument("--label-mode", type=int, default=3)
     ap.add_argument("--label-scale", type=float, default=1e-6)
+    ap.add_argument("--label-rel", type=float, default=0.1)
+    ap.add_argument("--label-floor", type=float, default=1e-14)
+    ap.add_argument("--y-floor", type=float, default=1e-12)
+    ap.add_argument("--trace", type=float, default=1e-6,
+                    help="extra reactors seeded with log-uniform species traces up to this (0: none)")
+    ap.add_argument("--trace-shape", type=int, nargs=3, default=(64, 64, 16))
     ap.add_argument("--elem-tol", type=float, default=1e30)
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--flow-steps", type=int, default=8,
@@ -107,6 +117,10 @@ def main(argv=None):
     t0 = time.time()
     log(f"generating data: {a.samples_shape} reactors x {a.records} records over {a.steps} steps")
     data = sur.generate(mech, tuple(a.samples_shape), a.steps, a.records, a.seed, log=log)
+    if a.trace > 0:
+        log(f"generating trace-seeded reactors: {a.trace_shape}, traces up to {a.trace:g}")
+        data = sur.concat(data, sur.generate(mech, tuple(a.trace_shape), a.steps, a.records,
+                                             a.seed + 7, trace=a.trace, log=log))
     if a.flow_steps > 0:
         log(f"generating flow data: {a.flow_size}^3 mixing layer, {a.flow_steps} steps")
         data = sur.concat(data, sur.generate_flow(mech, (a.flow_size,) * 3, a.flow_steps,
@@ -114,9 +128,11 @@ def main(argv=None):
     t_gen = time.time() - t0
     M = data.X.shape[0]
     log(f"  {M} samples in {t_gen:.1f}s, mean Newton iterations {data.newton_iters:.2f}")
-    cfg = sur.make_config(mech, data, res_tol=a.res_tol, res_T_tol=a.res_T_tol, dT_max=a.dT_max,
-                           chi_max=a.chi_max, label_scale=a.label_scale,
-                           elem_tol=a.elem_tol)
+    cfg = sur.make_config(mech, data, label_mode=a.label_mode, res_tol=a.res_tol,
+                           res_T_tol=a.res_T_tol, dT_max=a.dT_max, chi_max=a.chi_max,
+                           chain_max=a.chain_max,
+                           label_scale=a.label_scale, label_rel=a.label_rel,
+                           label_floor=a.label_floor, y_floor=a.y_floor, elem_tol=a.elem_tol)
     Xn, Yn = sur.normalize(cfg, data)
     del data
     # hold out 1/16 for validation
@@ -138,7 +154,8 @@ def main(argv=None):
     meta = {"samples": M, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
             "final_train_loss": hist[-1], "val_mse": val, "train_seconds": t_train,
             "generate_seconds": t_gen, "shape": list(a.samples_shape), "steps": a.steps,
-            "records": a.records, "seed": a.seed}
+            "records": a.records, "seed": a.seed, "label_mode": a.label_mode,
+            "trace": a.trace, "flow_steps": a.flow_steps}
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     sur.save(a.out, cfg, params, meta)
     ctx.close()

@@ -201,11 +201,28 @@ typedef struct {
                             chi = dt (d HRR / dT) / (rho c_v), HRR = -sum_k h_f,k W_k wdot_k,
                             above chi_max falls back (where the backward-Euler step can
                             jump between roots); <= 0: off */
+  float chain_max;       /* chain explosiveness at the predicted state: cells where the
+                            3x3 block of dt d wdot / d c over the chain carriers
+                            (chain_species) has an eigenvalue with real part >=
+                            chain_max fall back (near dt lambda = 1 the backward-Euler
+                            step of a radical-free hot mixture jumps to the ignited
+                            root while a near-root with a tiny residual exists);
+                            <= 0: off */
+  int32_t chain_species; /* bitmask of the (at most 3) chain-carrier species, bit k =
+                            species k (H2/O2: H, O, OH) */
   float boxcox_lambda;
   int32_t label_mode;    /* outputs: 0 dY_k; 1 d BoxCox(Y_k) (post_predict form,
                             Y' = (lambda (BoxCox(Y) + out) + 1)^(1/lambda));
-                            2 asinh(dY_k / label_scale_k) */
+                            2 asinh(dY_k / label_scale_k);
+                            3 asinh(dY_k / s_k) with the per-cell scale
+                            s_k = min(label_scale_k, max(label_floor, label_rel Y_k)):
+                            relative to Y_k for trace species, so a species at or
+                            near zero stays there (the element projection is then
+                            weighted per cell by (s_k + |dY_k|)^2) */
   float label_scale[9];  /* label_mode 2: dY_k = label_scale_k sinh(out_k) */
+  float label_rel;       /* label_mode 3 (see above) */
+  float label_floor;     /* label_mode 3 (see above) */
+  float y_floor;         /* inputs use BoxCox(max(Y_k, y_floor)); 0: BoxCox(Y_k) */
   int32_t n_elem;
   float elem[4][9];      /* atoms of element e in species k */
   double dt;             /* chemistry step the network predicts */
@@ -213,7 +230,8 @@ typedef struct {
 /* Allocates the network (parameters zero) and stores the configuration;
  * the context must have Ns = 9.  May be called again to change the config.
  * With n_elem > 0 every prediction is projected onto the element-conserving
- * subspace (weighted by out_std^2) after the element-balance gate. */
+ * subspace after the element-balance gate (weighted by out_std^2, or per cell
+ * in label_mode 3). */
 int f3d_nn_init(f3d_ctx* ctx, const f3d_nn_config* cfg);
 /* Host FP32 parameter vector of F3D_NN_NPARAM floats; set resets Adam. */
 int f3d_nn_set_params(f3d_ctx* ctx, const float* params);
@@ -235,7 +253,7 @@ int f3d_nn_forward(f3d_ctx* ctx, const float* X, float* Y, int32_t B, hipStream_
  * status [Nx*Ny*Nz] (device, may be NULL): 0 untouched, 1 surrogate; Newton
  * fallback after the gate that rejected the cell: 2 Mahalanobis distance,
  * 3 element balance, 4 positivity, 5 predicted temperature change,
- * 6 backward-Euler residual, 7 thermal explosiveness.
+ * 6 backward-Euler residual, 7 thermal explosiveness, 8 chain explosiveness.
  * *n_fallback (host) receives the number of fallback cells; this call
  * synchronizes the stream to size the fallback launch. */
 int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int32_t maxiter,

@@ -226,8 +226,14 @@ def test_nn_chem_all_fallback_equals_newton():
     ctx.close()
 
 
-def test_nn_chem_post_predict_and_projection():
-    print(f"\n[surrogate] nn_chem post_predict size={SIZE}", flush=True)
+@pytest.mark.parametrize("label_mode", [0, 3])
+def test_nn_chem_post_predict_and_projection(label_mode):
+    """With all weights zero the network outputs out_mean exactly; the
+    committed state must equal the host restatement of the label inversion,
+    the element projection (fixed out_std^2 weights in mode 0, per-cell
+    (s_k + |dY_k|)^2 weights in mode 3), post_predict and the FP64 energy
+    bookkeeping to 1e-12."""
+    print(f"\n[surrogate] nn_chem post_predict label_mode={label_mode} size={SIZE}", flush=True)
     abi = _abi()
     from oracle import Oracle
     m, Q, r, U = _cells(seed=21)
@@ -235,8 +241,16 @@ def test_nn_chem_post_predict_and_projection():
     cfg = abi.NNConfig()
     for k in range(11):
         cfg.in_std[k] = 1.0
-    mean = [-1.0e-4, -8.0e-4, 9.0e-4, 1.0e-5, 1.0e-5, 2.0e-5, 1.0e-6, 1.0e-7, 3.0e-5]
-    std = [1e-3, 2e-3, 2e-3, 1e-4, 1e-4, 3e-4, 1e-5, 1e-6, 1e-5]
+    if label_mode == 0:
+        mean = [-1.0e-4, -8.0e-4, 9.0e-4, 1.0e-5, 1.0e-5, 2.0e-5, 1.0e-6, 1.0e-7, 3.0e-5]
+        std = [1e-3, 2e-3, 2e-3, 1e-4, 1e-4, 3e-4, 1e-5, 1e-6, 1e-5]
+    else:
+        mean = [-2.0, -1.5, 2.5, 1.2, 0.7, 1.9, 0.4, -0.3, 0.6]  # asinh(dY / s) labels
+        std = [1.0] * 9
+        cfg.label_mode = 3
+        cfg.label_rel, cfg.label_floor = 0.1, 1e-14
+        for k in range(9):
+            cfg.label_scale[k] = 1e-6
     for k in range(9):
         cfg.out_mean[k], cfg.out_std[k] = mean[k], std[k]
     cfg.d_max, cfg.elem_tol, cfg.neg_tol, cfg.res_tol = 1e30, 1e30, 1.0, 0.0
@@ -253,17 +267,37 @@ def test_nn_chem_post_predict_and_projection():
     torch.cuda.synchronize()
     assert nfb == 0 and bool((st == 1).all())
     q1, r1, u1 = _interior(Q), _interior(r), _interior(U)
-    Pm = _projection(m, cfg)
-    dY = Pm @ np.array(mean, np.float32).astype(np.float64)
-    assert abs(dY[8]) < 1e-18  # N2 is inert: its increment is projected out
     comp = np.asarray(m.composition) / np.asarray(m.mw)[None, :]
-    assert np.max(np.abs(comp @ dY)) < 1e-12 * np.max(np.abs(comp @ np.abs(dY)))
+    o = np.array(mean, np.float32).astype(np.float64)
+    Pm = _projection(m, cfg)
+
+    def increment(Y):
+        if label_mode == 0:
+            return Pm @ o
+        f32 = [float(np.float32(v)) for v in (1e-6, 1e-14, 0.1)]
+        sk = np.minimum(f32[0], np.maximum(f32[1], f32[2] * Y))
+        d0 = sk * np.sinh(o)
+        S = (sk + np.abs(d0)) ** 2 + 1e-60
+        z = np.linalg.solve((comp * S[None, :]) @ comp.T, comp @ d0)
+        return d0 - S * (comp.T @ z)
+
+    # element conservation of the projected increment, to rounding of the
+    # projection: 1e-12 of the gross elemental turnover, or 1e-15 of the cell's
+    # elemental content (per-cell weights spanning 1e-28..1e-12 condition the
+    # 3x3 solve at ~1e8 in cells that hold only traces of a species class)
+    for c in (0, 100, 1000):
+        Y = np.maximum(r0[c] / q0[c, 0], 0)
+        dY = increment(Y)
+        assert abs(dY[8]) <= 1e-12 * np.max(np.abs(dY))  # N2 is inert: projected out
+        bar = 1e-12 * np.abs(comp) @ np.abs(dY) + 1e-15 * np.abs(comp) @ Y
+        assert np.all(np.abs(comp @ dY) <= bar), (c, comp @ dY, bar)
     orc = Oracle(*SIZE, m)
     lo6 = np.asarray(m.coeffs_lo)[5] * m.Ru / np.asarray(m.mw)
     worst = 0.0
     for c in range(0, q0.shape[0], 7):
         rho = q0[c, 0]
         Y = np.maximum(r0[c] / rho, 0)
+        dY = increment(Y)
         rn = np.maximum(Y + dY, 0) * rho
         dei = -np.sum(lo6 * (rn - r0[c]))
         rn = rn * (rho / rn.sum())
@@ -320,6 +354,67 @@ def test_nn_chem_residual_gate():
         if abs(R / tol - 1) < 1e-6:
             continue
         assert (s[c] == 1) == (R <= tol), (c, R, s[c])
+        checked += 1
+    assert checked > s.size // 4
+    ctx.close()
+
+
+def test_nn_chem_chain_gate():
+    """Identity prediction (dY = 0) on fresh radical-free reactors with only
+    the chain-explosiveness gate on: a cell falls back (status 8) exactly when
+    the 3x3 block dt d wdot / d c over H, O, OH at the predicted state (host
+    forward differences through the oracle's rates, eigenvalues by numpy) has
+    an eigenvalue with real part >= chain_max; rejected cells equal the
+    Newton step."""
+    print(f"\n[surrogate] nn_chem chain gate size={SIZE}", flush=True)
+    abi = _abi()
+    from oracle import Oracle
+    m, Q, r, U = _cells(seed=44)
+    ctx = abi.Context(*SIZE, m, "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    for k in range(9):
+        cfg.out_std[k] = 1.0
+    chain = [m.index("H"), m.index("O"), m.index("OH")]
+    cmax = 0.5
+    cfg.d_max, cfg.elem_tol, cfg.neg_tol, cfg.res_tol = 1e30, 1e30, 1.0, 0.0
+    cfg.chain_max, cfg.chain_species = cmax, sum(1 << k for k in chain)
+    cfg.boxcox_lambda, cfg.dt = 0.1, 1e-7
+    ctx.nn_init(cfg)
+    ctx.nn_set_params(np.zeros(12512, np.float32))  # identity prediction dY = 0
+    Qn, rn, Un = Q.clone(), r.clone(), U.clone()
+    ctx.chem_newton(Un, Qn, rn, 1e-7, 50, 1e-10, 1e-6)
+    q0, r0 = _interior(Q), _interior(r)
+    st = torch.zeros((SIZE[2], SIZE[1], SIZE[0]), dtype=torch.int8, device="cuda")
+    nfb = ctx.nn_chem(U, Q, r, 1e-7, 50, 1e-10, 1e-6, status=st)
+    torch.cuda.synchronize()
+    s = st.cpu().numpy().reshape(-1)
+    assert nfb == int((s >= 2).sum()) and 0 < nfb < s.size
+    assert set(np.unique(s).tolist()) <= {1, 8}  # only the chain gate is active
+    q1, r1, qn, rnn = _interior(Q), _interior(r), _interior(Qn), _interior(rn)
+    rej = s == 8
+    assert np.array_equal(q1[rej], qn[rej]) and np.array_equal(r1[rej], rnn[rej])
+    orc = Oracle(*SIZE, m)
+    mw = np.asarray(m.mw)
+    checked = 0
+    for c in range(0, s.size, 3):
+        rho = q0[c, 0]
+        rr = np.maximum(r0[c], 0)
+        rr = rr * (rho / rr.sum())
+        T, _ = orc.get_T(q0[c, 6], rr)
+        sc = rr / mw * 1e-6
+        w0 = orc.production_rate(sc, T, jac=False)
+        eps = 1e-8 * sc.sum()
+        M = np.zeros((3, 3))
+        for j, cj in enumerate(chain):
+            s2 = sc.copy()
+            s2[cj] += eps
+            M[:, j] = 1e-7 * (orc.production_rate(s2, T, jac=False) - w0)[chain] / eps
+        lam = np.linalg.eigvals(M).real.max()
+        if abs(lam / cmax - 1) < 1e-6:
+            continue
+        assert (s[c] == 1) == (lam < cmax), (c, lam, s[c])
         checked += 1
     assert checked > s.size // 4
     ctx.close()
