@@ -164,6 +164,113 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
   }
 }
 
+// Production rates (as production_rate without the Jacobian) together with
+// the 3x3 block M[i][j] = d wdot_{ci} / d c_{cj} over three chosen species
+// (ci < 0: unused row/column, left zero): exact mass-action derivatives
+// (finite at zero concentration) plus the [M] derivative of three-body and
+// falloff rates with the falloff factor held fixed (the Lindemann part).
+// One pass over the reactions: the transcendentals are shared with wdot.
+template <int NS>
+__device__ inline void production_rate_block3(const DevParams* __restrict__ P, const double* sc,
+                                              double T, const int (&ci)[3], double* wdot,
+                                              double (&M)[3][3]) {
+  const double lgT = log(T);
+  const double invT = 1.0 / T;
+  double gi[NS], cc[3];
+  gibbs<NS>(P, T, lgT, invT, gi);
+  const double RsT = P->Ru / P->atm * 1e6 * T;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) wdot[m] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) v += (m == ci[j]) ? sc[m] : 0.0;  // register select, no scratch
+    cc[j] = v;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) M[i][j] = 0.0;
+  }
+  const int Nr = P->Nr;
+  for (int n = 0; n < Nr; ++n) {
+    const double kf = P->Arr[n][0] * exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+    double dg = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
+    const double Kc = ipow(RsT, P->sgm[n]) * exp(dg);
+    const double kr = kf / Kc;
+    double q_f = kf;
+    double q_r = kr;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      if (vf != 0) q_f *= ipow(sc[m], vf);
+      if (vr != 0) q_r *= ipow(sc[m], vr);
+    }
+    const int rt = P->rtype[n];
+    double dlnq_dM = 0.0;
+    double fac = 1.0;
+    if (rt == 2) {
+      double mix = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      q_f *= mix;
+      q_r *= mix;
+      dlnq_dM = 1.0 / mix;
+      fac = mix;
+    } else if (rt == 3) {
+      double mix = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      const double redP = mix / kf * P->loP[n][0] * exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      const double logPred = log10(redP);
+      const double A_ = P->Troe[n][0];
+      const double logFcent = log10((1 - A_) * exp(-T / P->Troe[n][1]) +
+                                    A_ * exp(-T / P->Troe[n][2]) + exp(-P->Troe[n][3] * invT));
+      const double troe_c = -0.4 - 0.67 * logFcent;
+      const double troe_n = 0.75 - 1.27 * logFcent;
+      const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
+      const double F_troe = pow(10.0, logFcent / (1.0 + troe * troe));
+      const double k_1 = (redP / (1.0 + redP)) * F_troe;
+      q_f *= k_1;
+      q_r *= k_1;
+      dlnq_dM = 1.0 / ((1.0 + redP) * mix);
+      fac = k_1;
+    }
+    const double dq = q_f - q_r;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      if (vf != vr) wdot[m] += dq * (double)(vr - vf);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int cj = ci[j];
+      if (cj < 0) continue;
+      const int vfj = P->vf[n][cj], vrj = P->vr[n][cj];
+      double d = (rt != 1) ? dq * dlnq_dM * P->ef[n][cj] : 0.0;
+      if (vfj != 0 || vrj != 0) {
+        double pf = (vfj != 0) ? kf * (double)vfj * ipow(cc[j], vfj - 1) : 0.0;
+        double pr = (vrj != 0) ? kr * (double)vrj * ipow(cc[j], vrj - 1) : 0.0;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+          if (m == cj) continue;
+          const int vf = P->vf[n][m], vr = P->vr[n][m];
+          if (vf != 0) pf *= ipow(sc[m], vf);
+          if (vr != 0) pr *= ipow(sc[m], vr);
+        }
+        d += (pf - pr) * fac;
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int cr = ci[i];
+        if (cr < 0) continue;
+        const int dl = P->vr[n][cr] - P->vf[n][cr];
+        if (dl != 0) M[i][j] += d * (double)dl;
+      }
+    }
+  }
+}
+
 // solve, reactions.jl:463-503 (partial pivoting; row exchanges done with
 // selects so the matrix stays in registers).  Only the columns >= k that
 // the back substitution reads are eliminated; they are bit-identical to the

@@ -599,10 +599,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       //    and the temperature error its formation energies imply
       const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;
       if (res_on || C->chi_max > 0.0f || C->chain_max > 0.0f) {
-        double sc[NS], wdot[NS], Adum[NS][NS];
+        double sc[NS], wdot[NS], Adum[NS][NS], Mc[3][3];
 #pragma unroll
         for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
-        production_rate<NS, false>(P, sc, Tn, Adum, wdot);
+        if (C->chain_max > 0.0f) {
+          const int cidx[3] = {C->chain_idx[0], C->chain_idx[1], C->chain_idx[2]};
+          production_rate_block3<NS>(P, sc, Tn, cidx, wdot, Mc);
+        } else {
+          production_rate<NS, false>(P, sc, Tn, Adum, wdot);
+        }
         const double cvn = cv_mix<NS>(P, Tn, rn);
         if (res_on) {
           double rmax = 0.0, esum = 0.0;
@@ -627,32 +632,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
           if (!(dt * dq / hT <= (double)C->chi_max * cvn)) code = 7;
         }
         // 7. chain explosiveness: the 3x3 block M = dt d wdot / d c over the
-        //    chain carriers (forward differences, eps = 1e-8 of the total
-        //    concentration), rejected unless every eigenvalue of M has real
-        //    part < chain_max: Routh-Hurwitz on N = chain_max I - M
-        //    (tr N > 0, det N > 0, tr N * m2(N) > det N, m2 = principal minors)
+        //    chain carriers (from production_rate_block3 above), rejected
+        //    unless every eigenvalue of M has real part < chain_max:
+        //    Routh-Hurwitz on N = chain_max I - M (tr N > 0, det N > 0,
+        //    tr N * m2(N) > det N, m2 = sum of the principal 2x2 minors)
         if (code == 1 && C->chain_max > 0.0f) {
-          double ct = 0.0, Nm[3][3];
+          double Nm[3][3];
 #pragma unroll
-          for (int n = 0; n < NS; ++n) ct += sc[n];
-          const double eps = 1e-8 * ct;
+          for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const int cj = C->chain_idx[j];
-            double sc2[NS], w3[NS];
-#pragma unroll
-            for (int n = 0; n < NS; ++n) sc2[n] = sc[n] + (n == cj ? eps : 0.0);
-            if (cj >= 0) production_rate<NS, false>(P, sc2, Tn, Adum, w3);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const int ci = C->chain_idx[i];
-              double d = 0.0;
-#pragma unroll
-              for (int n = 0; n < NS; ++n) d += (n == ci) ? (w3[n] - wdot[n]) : 0.0;
-              const double Mij = (cj >= 0 && ci >= 0) ? dt * d / eps : 0.0;
-              Nm[i][j] = (i == j ? (double)C->chain_max : 0.0) - Mij;
-            }
-          }
+            for (int j = 0; j < 3; ++j)
+              Nm[i][j] = (i == j ? (double)C->chain_max : 0.0) - dt * Mc[i][j];
           const double tr = Nm[0][0] + Nm[1][1] + Nm[2][2];
           const double m2 = Nm[0][0] * Nm[1][1] - Nm[0][1] * Nm[1][0] +
                             Nm[0][0] * Nm[2][2] - Nm[0][2] * Nm[2][0] +

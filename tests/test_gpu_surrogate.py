@@ -362,10 +362,12 @@ def test_nn_chem_residual_gate():
 def test_nn_chem_chain_gate():
     """Identity prediction (dY = 0) on fresh radical-free reactors with only
     the chain-explosiveness gate on: a cell falls back (status 8) exactly when
-    the 3x3 block dt d wdot / d c over H, O, OH at the predicted state (host
-    forward differences through the oracle's rates, eigenvalues by numpy) has
-    an eigenvalue with real part >= chain_max; rejected cells equal the
-    Newton step."""
+    the 3x3 block dt d wdot / d c over H, O, OH at the predicted state has an
+    eigenvalue with real part >= chain_max; rejected cells equal the Newton
+    step.  The kernel differentiates the rates analytically (mass action and
+    [M], falloff factor held fixed); the host restatement takes forward
+    differences of the oracle's rates (relative step 1e-8) and numpy's
+    eigenvalues, so cells within 1e-3 of the threshold are not judged."""
     print(f"\n[surrogate] nn_chem chain gate size={SIZE}", flush=True)
     abi = _abi()
     from oracle import Oracle
@@ -412,7 +414,7 @@ def test_nn_chem_chain_gate():
             s2[cj] += eps
             M[:, j] = 1e-7 * (orc.production_rate(s2, T, jac=False) - w0)[chain] / eps
         lam = np.linalg.eigvals(M).real.max()
-        if abs(lam / cmax - 1) < 1e-6:
+        if abs(lam / cmax - 1) < 1e-3:
             continue
         assert (s[c] == 1) == (lam < cmax), (c, lam, s[c])
         checked += 1
