@@ -256,6 +256,43 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
   return t1;
 }
 
+// GetT warm-started from a nearby earlier temperature (the cell's T before
+// a stage or a chemistry step): the same Newton update, 1e-3 K tolerance,
+// +-100 K clamp and 30-update limit as get_T, from T_guess instead of the
+// reference's e / A_s start, and without the two bracketing energies.  A
+// guess outside [min_temp, max_temp] (or not finite), no convergence within
+// the 30 updates, or a result outside the range
+// (the reference's linear extrapolation region), takes get_T itself.  Both
+// stop within the tolerance of the same root, so they differ by < 1e-3 K.
+template <int NS>
+__device__ inline double get_T_from(const DevParams* P, double ein, const double* rhoi,
+                                    double T_guess) {
+  const double tol = 1e-3;
+  const double tmin = P->min_temp, tmax = P->max_temp;
+  if (!(T_guess >= tmin && T_guess <= tmax)) return get_T<NS>(P, ein, rhoi, nullptr);
+  double t1 = T_guess;
+  bool done = false;
+  for (int m = 0; m < 30; ++m) {
+    const double e1 = internal_energy<NS>(P, t1, rhoi);
+    const double cv = cv_mix<NS>(P, t1, rhoi);
+    double dt = (ein - e1) / cv;
+    if (dt > 100.0) {
+      dt = 100.0;
+    } else if (dt < -100.0) {
+      dt = -100.0;
+    } else if (fabs(dt) < tol) {
+      done = true;
+      break;
+    } else if (t1 + dt == t1) {
+      done = true;
+      break;
+    }
+    t1 += dt;
+  }
+  if (!done || !(t1 >= tmin && t1 <= tmax)) return get_T<NS>(P, ein, rhoi, nullptr);
+  return t1;
+}
+
 // common epilogue of eval_gpu / eval_gpu_stiff (reactions.jl:209-229, :286-306)
 template <int NS>
 __device__ inline void chem_epilogue(const DevParams* __restrict__ P, double* __restrict__ U,

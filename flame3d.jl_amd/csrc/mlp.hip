@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
       for (int n = 0; n < NS; ++n) rn[n] *= rhoc / sr;
       const double ein = Q[p + 6 * NP] + dei;
-      const double Tn = fmax(get_T<NS>(P, ein, rn, nullptr), kEps);
+      const double Tn = fmax(get_T_from<NS>(P, ein, rn, T), kEps);
       // 5. backward-Euler residual at (rho', T'): its largest species entry,
       //    and the temperature error its formation energies imply
       const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;

@@ -200,8 +200,9 @@ __global__ void __launch_bounds__(kBlock)
   mu[p] = m;
 }
 
-// c2Prim, utils.jl:2-48 (FP64)
-template <int NS>
+// c2Prim, utils.jl:2-48 (FP64).  WARM (FP32-flux contexts): the e -> T
+// Newton starts from the cell's previous temperature Q[5] (get_T_from).
+template <int NS, bool WARM>
 __global__ void __launch_bounds__(kBlock)
     k_c2prim(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
              const int64_t* __restrict__ tag, const DevParams* __restrict__ P, Geom g) {
@@ -227,7 +228,8 @@ __global__ void __launch_bounds__(kBlock)
   const double v = U[p + 2 * NP] * rinv;
   const double w = U[p + 3 * NP] * rinv;
   const double ei = fmax(U[p + 4 * NP] - 0.5 * rho * (u * u + v * v + w * w), kEps);
-  const double T = fmax(get_T<NS>(P, ei, r, nullptr), kEps);
+  const double T = fmax(WARM ? get_T_from<NS>(P, ei, r, Q[p + 5 * NP]) : get_T<NS>(P, ei, r, nullptr),
+                       kEps);
   const double pr = fmax(pmixture<NS>(P, T, r), kEps);
 #pragma unroll
   for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = r[n];
@@ -388,12 +390,21 @@ int launch_mixture(f3d_ctx* c, const double* Q, const double* rhoi, void* Yi, vo
 int launch_c2prim(f3d_ctx* c, double* U, double* Q, double* rhoi, const int64_t* tag,
                   hipStream_t s) {
   const long n = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+  const bool warm = c->precision == F3D_FP32;
+#define F3D_C2P(NS_)                                                                         \
+  if (warm)                                                                                  \
+    hipLaunchKernelGGL((k_c2prim<NS_, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, \
+                       tag, c->dparams, c->g);                                               \
+  else                                                                                       \
+    hipLaunchKernelGGL((k_c2prim<NS_, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q,      \
+                       rhoi, tag, c->dparams, c->g)
   switch (c->g.Ns) {
-    case 1: hipLaunchKernelGGL(k_c2prim<1>, dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, tag, c->dparams, c->g); break;
-    case 5: hipLaunchKernelGGL(k_c2prim<5>, dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, tag, c->dparams, c->g); break;
-    case 9: hipLaunchKernelGGL(k_c2prim<9>, dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, tag, c->dparams, c->g); break;
+    case 1: F3D_C2P(1); break;
+    case 5: F3D_C2P(5); break;
+    case 9: F3D_C2P(9); break;
     default: set_error("c2prim: Ns must be 1, 5 or 9"); return F3D_ERR_UNSUPPORTED;
   }
+#undef F3D_C2P
   return hip_check(hipGetLastError(), "c2prim");
 }
 

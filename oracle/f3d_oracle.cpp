@@ -215,6 +215,38 @@ double or_get_T(const or_dims* d, const or_thermo* th, double ein, const double*
   return t1;
 }
 
+// GetT warm-started from T_guess (the library's get_T_from, used on the FP32
+// stage's c2Prim and by the surrogate step): the Newton update of GetT from
+// T_guess; a guess or result outside [tmin, tmax] takes or_get_T.  Parity
+// unpinned against the reference (it has no warm start); pinned here by
+// agreement with or_get_T to the 1e-3 K tolerance (tests/).
+double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const double* rhoi,
+                     double T_guess, double tol, int32_t maxiter, double tmin, double tmax) {
+  if (!(T_guess >= tmin && T_guess <= tmax))
+    return or_get_T(d, th, ein, rhoi, tol, maxiter, tmin, tmax, nullptr);
+  double t1 = T_guess;
+  bool done = false;
+  for (int m = 0; m < maxiter; ++m) {
+    const double e1 = or_internal_energy(d, th, t1, rhoi);
+    const double cv = or_cv(d, th, t1, rhoi);
+    double dt = (ein - e1) / cv;
+    if (dt > 100.0) {
+      dt = 100.0;
+    } else if (dt < -100.0) {
+      dt = -100.0;
+    } else if (std::fabs(dt) < tol) {
+      done = true;
+      break;
+    } else if (t1 + dt == t1) {
+      done = true;
+      break;
+    }
+    t1 += dt;
+  }
+  if (!done || !(t1 >= tmin && t1 <= tmax)) return or_get_T(d, th, ein, rhoi, tol, maxiter, tmin, tmax, nullptr);
+  return t1;
+}
+
 static double getT_ref(const Grid& g, const or_dims* d, const or_thermo* th, double ein,
                        const double* rhoi) {
   (void)g;

@@ -97,6 +97,7 @@ class Oracle:
                               (C.c_double * 7)(*s["UP7"]), s["T_criteria"])
         L = self.lib
         L.or_get_T.restype = C.c_double
+        L.or_get_T_from.restype = C.c_double
         L.or_internal_energy.restype = C.c_double
         L.or_cv.restype = C.c_double
         L.or_num_threads.restype = C.c_int32
@@ -127,6 +128,13 @@ class Oracle:
     def cv(self, T, rhoi):
         return self.lib.or_cv(C.byref(self.dims), C.byref(self.thermo), C.c_double(T),
                               _f(np.ascontiguousarray(rhoi, dtype=np.float64)))
+
+    def get_T_from(self, e, rhoi, T_guess, tol=1e-3, maxiter=30):
+        r = np.ascontiguousarray(rhoi, dtype=np.float64)
+        return self.lib.or_get_T_from(C.byref(self.dims), C.byref(self.thermo), C.c_double(e),
+                                      _f(r), C.c_double(T_guess), C.c_double(tol),
+                                      C.c_int32(maxiter), C.c_double(self.mech.min_temp),
+                                      C.c_double(self.mech.max_temp))
 
     def get_T(self, e, rhoi, tol=1e-3, maxiter=30, tmin=None, tmax=None):
         it = C.c_int32(0)

@@ -302,7 +302,7 @@ def test_nn_chem_post_predict_and_projection(label_mode):
         dei = -np.sum(lo6 * (rn - r0[c]))
         rn = rn * (rho / rn.sum())
         ein = q0[c, 6] + dei
-        T, _ = orc.get_T(ein, rn)
+        T = orc.get_T_from(ein, rn, q0[c, 5])  # the kernel warm-starts GetT at the old T
         p = m.Ru * T * np.sum(rn / np.asarray(m.mw))
         e_sp = np.max(np.abs(r1[c] - rn)) / rho
         exp = np.array([T, p, ein, u0[c, 4] + dei])
@@ -348,7 +348,7 @@ def test_nn_chem_residual_gate():
         rho = q0[c, 0]
         rr = np.maximum(r0[c], 0)
         rr = rr * (rho / rr.sum())
-        T, _ = orc.get_T(q0[c, 6], rr)
+        T = orc.get_T_from(q0[c, 6], rr, q0[c, 5])
         w = orc.production_rate(rr / mw * 1e-6, T, jac=False)
         R = np.max(np.abs(rr - r0[c] - w * mw * 1e6 * 1e-7)) / rho
         if abs(R / tol - 1) < 1e-6:
@@ -404,7 +404,7 @@ def test_nn_chem_chain_gate():
         rho = q0[c, 0]
         rr = np.maximum(r0[c], 0)
         rr = rr * (rho / rr.sum())
-        T, _ = orc.get_T(q0[c, 6], rr)
+        T = orc.get_T_from(q0[c, 6], rr, q0[c, 5])
         sc = rr / mw * 1e-6
         w0 = orc.production_rate(sc, T, jac=False)
         eps = 1e-8 * sc.sum()

@@ -347,6 +347,31 @@ def test_getT_inverts_energy_and_extrapolates(h2):
     assert abs(T - (300.0 - 1000.0)) < 1e-9 and it == 0
 
 
+def test_getT_warm_start_agrees_with_getT(h2):
+    """GetT warm-started from a nearby temperature (the FP32 stage's c2Prim and
+    the surrogate step use it) stops within the 1e-3 K tolerance of the same
+    root as the reference start; outside [min_temp, max_temp] it takes the
+    reference's extrapolation exactly; a guess outside the range falls back."""
+    from oracle import Oracle
+    orc = Oracle(1, 1, 1, h2)
+    rng = np.random.default_rng(6)
+    for _ in range(200):
+        r = rng.uniform(0, 1, h2.Ns)
+        T0 = rng.uniform(350, 4800)
+        e = orc.internal_energy(T0, r)
+        for guess in (T0 + rng.uniform(-30, 30), T0 + rng.uniform(-400, 400), 1000.0):
+            guess = min(max(guess, 300.0), 5000.0)
+            Tw = orc.get_T_from(e, r, guess)
+            assert abs(Tw - T0) < 1e-3, (T0, guess, Tw)
+            assert abs(Tw - orc.get_T(e, r)[0]) < 2e-3
+    r = rng.uniform(0, 1, h2.Ns)
+    e = orc.internal_energy(300.0, r) - 1000.0 * orc.cv(300.0, r)
+    assert orc.get_T_from(e, r, 400.0) == orc.get_T(e, r)[0]  # extrapolation region
+    e = orc.internal_energy(1500.0, r)
+    assert orc.get_T_from(e, r, float("nan")) == orc.get_T(e, r)[0]
+    assert orc.get_T_from(e, r, 9000.0) == orc.get_T(e, r)[0]
+
+
 def test_gibbs_equals_h_minus_s(h2):
     """g/RT from thermo.jl:163-191 equals h/RT - s/R of the NASA-7 fit."""
     from oracle import Oracle
