@@ -83,7 +83,11 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// shockSensor, schemes.jl:1-24 (0-based range [1, N+2NG-2] per direction)
+// shockSensor, schemes.jl:1-24 (0-based range [1, N+2NG-2] per direction).
+// Evaluated in FP64 on both paths and stored in R: the second difference of p
+// cancels (p ~ 1e5-1e6 Pa against differences of ~1e2), which in float
+// arithmetic would move phi by ~1e-4 relative and flip the UP7/WENO/NND branch
+// at faces near a threshold.
 template <typename R>
 __global__ void __launch_bounds__(kBlock)
     k_shock_sensor(R* __restrict__ phi, const double* __restrict__ Q, Geom g) {
@@ -95,13 +99,13 @@ __global__ void __launch_bounds__(kBlock)
   if (i < 1 || i > g.Nxt - 2 || j < 1 || j > g.Nyt - 2 || k < 1 || k > g.Nzt - 2) return;
   const double* Pp = Q + 4L * g.NP;
   const long sy = g.Nxt, sz = (long)g.Nxt * g.Nyt;
-  const R Px1 = (R)Pp[p - 1], Px2 = (R)Pp[p], Px3 = (R)Pp[p + 1];
-  const R Py1 = (R)Pp[p - sy], Py3 = (R)Pp[p + sy];
-  const R Pz1 = (R)Pp[p - sz], Pz3 = (R)Pp[p + sz];
-  const R px = fabs(-Px1 + R(2) * Px2 - Px3) / (Px1 + R(2) * Px2 + Px3);
-  const R py = fabs(-Py1 + R(2) * Px2 - Py3) / (Py1 + R(2) * Px2 + Py3);
-  const R pz = fabs(-Pz1 + R(2) * Px2 - Pz3) / (Pz1 + R(2) * Px2 + Pz3);
-  phi[p] = px + py + pz;
+  const double Px1 = Pp[p - 1], Px2 = Pp[p], Px3 = Pp[p + 1];
+  const double Py1 = Pp[p - sy], Py3 = Pp[p + sy];
+  const double Pz1 = Pp[p - sz], Pz3 = Pp[p + sz];
+  const double px = fabs(-Px1 + 2.0 * Px2 - Px3) / (Px1 + 2.0 * Px2 + Px3);
+  const double py = fabs(-Py1 + 2.0 * Px2 - Py3) / (Py1 + 2.0 * Px2 + Py3);
+  const double pz = fabs(-Pz1 + 2.0 * Px2 - Pz3) / (Pz1 + 2.0 * Px2 + Pz3);
+  phi[p] = (R)(px + py + pz);
 }
 
 // WENO5-Z weights alpha_k = d_k (1 + (tau / (eps + beta_k))^2), d = (1, 6, 3)

@@ -249,7 +249,7 @@ def floor_inputs(X, y_floor: float, lam: float = LAMBDA):
 
 def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = LABEL_SCALE,
                 elem_tol: float = 1e30, neg_tol: float = 1e-6, res_tol: float = 3e-5,
-                res_T_tol: float = 0.25, dT_max: float = 20.0, chi_max: float = 0.05,
+                res_T_tol: float = 0.25, dT_max: float = 20.0, chi_max: float = 0.0,
                 chain_max: float = CHAIN_MAX, chain_species=CHAIN_SPECIES,
                 label_rel: float = LABEL_REL, label_floor: float = LABEL_FLOOR,
                 y_floor: float = Y_FLOOR, sigmas: float = 3.0,

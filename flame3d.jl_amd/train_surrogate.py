@@ -84,7 +84,7 @@ def main(argv=None):
     ap.add_argument("--res-tol", type=float, default=3e-5)
     ap.add_argument("--res-T-tol", type=float, default=0.25)
     ap.add_argument("--dT-max", type=float, default=20.0)
-    ap.add_argument("--chi-max", type=float, default=0.05)
+    ap.add_argument("--chi-max", type=float, default=0.0)
     ap.add_argument("--chain-max", type=float, default=0.5)
     ap.add_argument("--label-mode", type=int, default=3)
     ap.add_argument("--label-scale", type=float, default=1e-6)
