@@ -264,6 +264,10 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
 // the 30 updates, or a result outside the range
 // (the reference's linear extrapolation region), takes get_T itself.  Both
 // stop within the tolerance of the same root, so they differ by < 1e-3 K.
+// Unlike get_T it also stops when the update flips sign without shrinking:
+// an energy inside the small jump of the NASA-7 fits at their breakpoint
+// (e.g. a stream at 1000 K) has no exact root, and get_T then spends its 30
+// updates oscillating across the breakpoint.
 template <int NS>
 __device__ inline double get_T_from(const DevParams* P, double ein, const double* rhoi,
                                     double T_guess) {
@@ -272,6 +276,7 @@ __device__ inline double get_T_from(const DevParams* P, double ein, const double
   if (!(T_guess >= tmin && T_guess <= tmax)) return get_T<NS>(P, ein, rhoi, nullptr);
   double t1 = T_guess;
   bool done = false;
+  double dt_prev = 0.0;
   for (int m = 0; m < 30; ++m) {
     const double e1 = internal_energy<NS>(P, t1, rhoi);
     const double cv = cv_mix<NS>(P, t1, rhoi);
@@ -286,7 +291,13 @@ __device__ inline double get_T_from(const DevParams* P, double ein, const double
     } else if (t1 + dt == t1) {
       done = true;
       break;
+    } else if (dt * dt_prev < 0.0 && fabs(dt) >= 0.5 * fabs(dt_prev)) {
+      // the update flips sign without shrinking: the root sits in the small
+      // energy jump of the NASA-7 fits at their breakpoint (no exact root)
+      done = true;
+      break;
     }
+    dt_prev = dt;
     t1 += dt;
   }
   if (!done || !(t1 >= tmin && t1 <= tmax)) return get_T<NS>(P, ein, rhoi, nullptr);

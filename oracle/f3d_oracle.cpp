@@ -226,6 +226,7 @@ double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const do
     return or_get_T(d, th, ein, rhoi, tol, maxiter, tmin, tmax, nullptr);
   double t1 = T_guess;
   bool done = false;
+  double dt_prev = 0.0;
   for (int m = 0; m < maxiter; ++m) {
     const double e1 = or_internal_energy(d, th, t1, rhoi);
     const double cv = or_cv(d, th, t1, rhoi);
@@ -240,7 +241,13 @@ double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const do
     } else if (t1 + dt == t1) {
       done = true;
       break;
+    } else if (dt * dt_prev < 0.0 && std::fabs(dt) >= 0.5 * std::fabs(dt_prev)) {
+      // the update flips sign without shrinking: the root sits in the small
+      // energy jump of the NASA-7 fits at their breakpoint (no exact root)
+      done = true;
+      break;
     }
+    dt_prev = dt;
     t1 += dt;
   }
   if (!done || !(t1 >= tmin && t1 <= tmax)) return or_get_T(d, th, ein, rhoi, tol, maxiter, tmin, tmax, nullptr);

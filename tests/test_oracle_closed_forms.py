@@ -370,6 +370,18 @@ def test_getT_warm_start_agrees_with_getT(h2):
     e = orc.internal_energy(1500.0, r)
     assert orc.get_T_from(e, r, float("nan")) == orc.get_T(e, r)[0]
     assert orc.get_T_from(e, r, 9000.0) == orc.get_T(e, r)[0]
+    # an energy inside the NASA-7 jump at 1000 K (H2-rich: ~0.2 K wide) has no
+    # root; the reference start spends its 30 updates oscillating across the
+    # breakpoint, the warm start stops at the first non-shrinking flip, both
+    # inside the jump
+    r = np.zeros(h2.Ns)
+    r[h2.index("H2")], r[h2.index("N2")] = 0.3, 0.01
+    elo, ehi = orc.internal_energy(1000.0 - 1e-9, r), orc.internal_energy(1000.0, r)
+    gap = (ehi - elo) / orc.cv(1000.0, r)
+    assert gap > 1e-2
+    T_ref, it = orc.get_T(0.5 * (elo + ehi), r)
+    assert it == 30 and abs(T_ref - 1000.0) <= gap
+    assert abs(orc.get_T_from(0.5 * (elo + ehi), r, 1000.3) - 1000.0) <= gap
 
 
 def test_gibbs_equals_h_minus_s(h2):
