@@ -191,15 +191,20 @@ def time_kernels(ctx, torch, Q, rhoi, metrics, J, reps=10):
         for fn in fns:
             fn()
         torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            for fn in fns:
-                fn()
-        e1.record()
-        torch.cuda.synchronize()
-        out[name] = (e0.elapsed_time(e1) / (reps * len(fns)), len(fns))
+        # the fastest of five timed batches: a host stall while the launches
+        # are enqueued (the kernels take ~0.3 ms) would otherwise count as GPU time
+        best = float("inf")
+        for _ in range(5):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                for fn in fns:
+                    fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) / (reps * len(fns)))
+        out[name] = (best, len(fns))
     return out
 
 

@@ -256,21 +256,30 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // y and z sweeps: one thread per (x, transverse) line marches through a
-// segment of faces along the sweep, holding the eight-point stencil of Fp and
-// Fm in registers, so each point is read once per segment (plus a 7-point
-// halo) with x-coalesced loads; the face branches are found once, from phi,
-// and kept as 2-bit codes (3 = tagged face, skipped) for every variable.
+// segment of faces along the sweep for one variable, holding the eight-point
+// stencil of Fp and Fm in registers, so each point is read once per segment
+// (plus a 7-point halo) with x-coalesced loads; the face branches are found
+// from phi and kept as 2-bit codes (3 = tagged face, skipped).  One wave per
+// (64 x, transverse line, segment, variable): the variables of a segment are
+// consecutive blocks after the XCD remap, so they share the phi rows in L2,
+// and the launch has NV times the waves of a wave-per-segment march (a march
+// is latency-bound: its waves' chunks of loads are the parallelism).
 constexpr int kMarchSeg = 32;
 template <typename R>
 __global__ void __launch_bounds__(64)
     k_weno_march(R* __restrict__ F, const R* __restrict__ phi, const R* __restrict__ Fp,
                  const R* __restrict__ Fm, int NV, const int64_t* __restrict__ tag,
                  const DevParams* __restrict__ P, Geom g, int dir, int seg) {
-  const int a = blockIdx.x * 64 + threadIdx.x;
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned L = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+  const int var = (int)(L % (unsigned)NV);
+  const unsigned rest = L / (unsigned)NV;
+  const int a = (int)(rest % gridDim.x) * 64 + threadIdx.x;
   if (a >= g.Nx) return;
-  const int tr = blockIdx.y;  // z index (y sweep) or y index (z sweep)
+  const int tr = (int)((rest / gridDim.x) % gridDim.y);  // z index (y sweep) or y index (z sweep)
+  const int sgi = (int)(rest / (gridDim.x * gridDim.y));
   const int nd = (dir == 1 ? g.Ny : g.Nz) + 1;  // faces along the sweep
-  const int j0 = blockIdx.z * seg;
+  const int j0 = sgi * seg;
   const int j1 = min(j0 + seg, nd);
   if (j0 >= j1) return;
   const long s = (dir == 1) ? (long)g.Nxt : (long)g.Nxt * g.Nyt;
@@ -298,7 +307,8 @@ __global__ void __launch_bounds__(64)
   // together (independent loads in flight), rows past the array clamped
   constexpr int kCh = 8;
   const long rmax = base + (long)((dir == 1 ? g.Nyt : g.Nzt) - 1) * s;
-  for (int n = 0; n < NV; ++n) {
+  {
+    const int n = var;
     const R* fp = Fp + (long)n * g.NP;
     const R* fm = Fm + (long)n * g.NP;
     R pw[kCh + 7], mw[kCh + 7];  // rows j-4 .. j+kCh+2 of the chunk starting at face j
@@ -405,7 +415,7 @@ int launch_weno(f3d_ctx* c, int dir, void* F, const void* phi, const void* Fp, c
     const int nd = (dir == 1 ? g.Ny : g.Nz) + 1;
     const int nseg = (nd + kMarchSeg - 1) / kMarchSeg;
     const int seg = (nd + nseg - 1) / nseg;  // <= kMarchSeg: the branch codes fit 64 bits
-    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg);
+    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg * NV);
     if (c->precision == F3D_FP64)
       hipLaunchKernelGGL(k_weno_march<double>, grid, dim3(64), 0, s, (double*)F,
                          (const double*)phi, (const double*)Fp, (const double*)Fm, NV, tag,
