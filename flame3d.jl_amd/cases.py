@@ -372,10 +372,22 @@ def reacting_mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", 
     return Case("reacting_mixing_layer", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
 
 
+def tgv_gas(Ma=0.1, Re=1600.0, T0=300.0, P0=101325.0, R_gas=287.0, gamma=1.4, Pr=0.72):
+    """The calorically perfect gas of the Taylor-Green case (gamma 1.4, Pr 0.72)
+    with the constant viscosity that gives Re = rho0 V0 L / mu at L = 1 m."""
+    from . import mech as mm
+    c0 = math.sqrt(gamma * R_gas * T0)
+    rho0 = P0 / (R_gas * T0)
+    return mm.perfect_gas(R_gas=R_gas, gamma=gamma, mu=rho0 * Ma * c0 / Re, Pr=Pr)
+
+
 def taylor_green(N=64, mech=None, Ma=0.1, Re=1600.0, T0=300.0, cfl=0.5, dev="cpu") -> Case:
     """Taylor-Green vortex on the periodic [0, 2 pi L)^3 box with V0 = Ma c0,
-    rho0 from p0 = rho0 R T0, mu from Re = rho0 V0 L / mu (L = 1 m)."""
+    rho0 from p0 = rho0 R T0, mu from Re = rho0 V0 L / mu (L = 1 m); mech
+    None: tgv_gas(Ma, Re, T0)."""
     from . import abi
+    if mech is None:
+        mech = tgv_gas(Ma, Re, T0)
     L = 1.0
     grid = uniform_grid(N, N, N, 2 * math.pi * L, 2 * math.pi * L, 2 * math.pi * L, dev=dev)
     W = float(mech.mw[0])
