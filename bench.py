@@ -240,13 +240,13 @@ def strang_step(ctx, st, dt, chem):
 
 
 def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate=None,
-             chemistry=True):
+             chemistry=True, scheme=None):
     """Time `steps` full time steps of the path.  prec fp64: FP64 stage +
     FP64 backward-Euler Newton chemistry (the plain method); fp32 with a
     surrogate: FP32 flux arithmetic + the BF16 MLP chemistry step with Newton
     fallback.  Each chemistry call is also timed on its own (HIP events)."""
     N = case.grid.Nx, case.grid.Ny, case.grid.Nz
-    ctx = abi.Context(*N, mech, prec)
+    ctx = abi.Context(*N, mech, prec, scheme=scheme)
     Q = case.Q.clone()
     rhoi = case.rhoi.clone()
     U = ctx.zeros(5)
@@ -360,17 +360,18 @@ def cpu_baseline(mech, size, steps, min_seconds=10.0):
 
 
 def bench_air(a, torch, abi, cases, mm):
-    """The shock-laden mixing layer of 5-species air, non-reacting: one RK3
-    stage (the step) on the FP32 path against the FP64 path."""
+    """The shock-laden mixing layer of 5-species air, non-reacting, hybrid
+    WENO5-Z / UP7 switched by the Ducros sensor at 0.1: one RK3 stage (the
+    step) on the FP32 path against the FP64 path."""
     mech = mm.load("air5")
     N = (a.size, a.size, a.size)
     case = cases.mixing_layer(N, mech, dev="cuda")
     nc = N[0] * N[1] * N[2]
     stages = 3 * a.steps
     c64, r64 = run_path("fp64", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
-                        chemistry=False)
+                        chemistry=False, scheme=abi.DUCROS_SCHEME)
     c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
-                        chemistry=False)
+                        chemistry=False, scheme=abi.DUCROS_SCHEME)
     ms64, ms32 = r64["ms_total"] / stages, r32["ms_total"] / stages
     errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
     npts = (N[0] + 8) * (N[1] + 8) * (N[2] + 8)
@@ -388,7 +389,8 @@ def bench_air(a, torch, abi, cases, mm):
         "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": stages,
         "warmup": a.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 flux, f64 state", "data": "synthetic",
-        "config": {"workload": f"shock-laden mixing layer {N[0]}^3, 5-species air, non-reacting"},
+        "config": {"workload": f"shock-laden mixing layer {N[0]}^3, 5-species air, non-reacting, "
+                               "WENO5-Z/UP7 switched by the Ducros sensor at 0.1"},
         "fp64": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64},
         "speedup_fp32_vs_fp64": ms64 / ms32, "error_fp32_vs_fp64": errs, "kernels": ktab}))
     c32.close()

@@ -203,6 +203,10 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
     set_error("f3d_create: precision must be F3D_FP64 or F3D_FP32");
     return F3D_ERR_ARG;
   }
+  if (sc->sensor_kind != F3D_SENSOR_PRESSURE && sc->sensor_kind != F3D_SENSOR_DUCROS) {
+    set_error("f3d_create: sensor_kind must be F3D_SENSOR_PRESSURE or F3D_SENSOR_DUCROS");
+    return F3D_ERR_ARG;
+  }
   f3d_ctx* c = new (std::nothrow) f3d_ctx();
   if (!c) {
     set_error("f3d_create: out of host memory");
@@ -219,6 +223,7 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
   c->g.NP = (int)NP;
   c->precision = sc->precision;
   c->bug_compat = sc->bug_compat;
+  c->sensor_kind = sc->sensor_kind;
   int rc = fill_params(c->host_params, d, th, kin, sc);
   if (rc) {
     delete c;

@@ -49,6 +49,7 @@ struct f3d_ctx {
   f3d::Geom g;
   int precision;
   int bug_compat;
+  int sensor_kind = 0;  // F3D_SENSOR_PRESSURE / F3D_SENSOR_DUCROS
   f3d::DevParams host_params;
   f3d::DevParams* dparams = nullptr;
   // boundary fills

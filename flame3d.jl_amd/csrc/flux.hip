@@ -108,6 +108,42 @@ __global__ void __launch_bounds__(kBlock)
   phi[p] = (R)(px + py + pz);
 }
 
+// Ducros sensor (no reference counterpart; the reference switches on the
+// pressure sensor above): theta = (div u)^2 / ((div u)^2 + |curl u|^2 + 1e-30),
+// velocity gradients from 2nd-order central differences along xi, eta, zeta
+// and the metrics (d xi_a / d x_j = M[3a + j] J, the J^-1-weighted cofactors),
+// in FP64 on both paths, on the points shockSensor covers.
+struct Met9 {
+  const double* m[9];
+};
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_ducros(R* __restrict__ phi, const double* __restrict__ Q, Met9 M,
+             const double* __restrict__ J, Geom g) {
+  const long p = gid();
+  if (p >= g.NP) return;
+  const int i = (int)(p % g.Nxt);
+  const int t = (int)(p / g.Nxt);
+  const int j = t % g.Nyt, k = t / g.Nyt;
+  if (i < 1 || i > g.Nxt - 2 || j < 1 || j > g.Nyt - 2 || k < 1 || k > g.Nzt - 2) return;
+  const long st[3] = {1, (long)g.Nxt, (long)g.Nxt * g.Nyt};
+  const double Jp = J[p];
+  double G[3][3];  // G[c][x] = d u_c / d x_x
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* u = Q + (long)(1 + c) * g.NP;
+    double dxi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dxi[a] = 0.5 * (u[p + st[a]] - u[p - st[a]]);
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+      G[c][x] = Jp * (M.m[x][p] * dxi[0] + M.m[3 + x][p] * dxi[1] + M.m[6 + x][p] * dxi[2]);
+  }
+  const double dv = G[0][0] + G[1][1] + G[2][2];
+  const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+  phi[p] = (R)(dv * dv / (dv * dv + wx * wx + wy * wy + wz * wz + 1e-30));
+}
+
 // WENO5-Z weights alpha_k = d_k (1 + (tau / (eps + beta_k))^2), d = (1, 6, 3)
 // (schemes.jl:152-155).  FP64 evaluates the reference expression as written.
 // In FP32, eps = 1e-26 makes (tau/(eps+beta_k))^2 overflow float whenever a
@@ -398,6 +434,21 @@ int launch_to_float(f3d_ctx* c, float* dst, const double* src, long n, hipStream
 
 int launch_shock_sensor(f3d_ctx* c, void* phi, const double* Q, hipStream_t s) {
   const int NP = c->g.NP;
+  if (c->sensor_kind == F3D_SENSOR_DUCROS) {
+    if (!c->J) {
+      set_error("shock_sensor: the Ducros sensor needs the metrics (f3d_set_metrics first)");
+      return F3D_ERR_ARG;
+    }
+    Met9 M;
+    for (int q = 0; q < 9; ++q) M.m[q] = c->metrics[q];
+    if (c->precision == F3D_FP64)
+      hipLaunchKernelGGL(k_ducros<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s, (double*)phi, Q,
+                         M, c->J, c->g);
+    else
+      hipLaunchKernelGGL(k_ducros<float>, dim3(nblocks(NP)), dim3(kBlock), 0, s, (float*)phi, Q,
+                         M, c->J, c->g);
+    return hip_check(hipGetLastError(), "shock_sensor");
+  }
   if (c->precision == F3D_FP64)
     hipLaunchKernelGGL(k_shock_sensor<double>, dim3(nblocks(NP)), dim3(kBlock), 0, s,
                        (double*)phi, Q, c->g);

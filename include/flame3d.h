@@ -89,8 +89,15 @@ typedef struct {
   int32_t precision;   /* F3D_FP64 or F3D_FP32 (flux/viscous/divergence arithmetic) */
   int32_t bug_compat;  /* bit 0: eval_gpu_stiff's transposed coeffs_lo[n,6] read
                           (reactions.jl:282), valid for Ns >= 6 */
-  int32_t reserved[6];
+  int32_t sensor_kind; /* F3D_SENSOR_PRESSURE: the reference's shockSensor (schemes.jl:1-24);
+                          F3D_SENSOR_DUCROS: theta = (div u)^2 / ((div u)^2 + |curl u|^2 + 1e-30)
+                          from central differences and the metrics (needs f3d_set_metrics);
+                          theta in [0, 1] then drives the same switch: hybrid_phi1 = 0.1,
+                          hybrid_phi2 > 1 gives UP7 below 0.1 and WENO5-Z above */
+  int32_t reserved[5];
 } f3d_scheme;
+#define F3D_SENSOR_PRESSURE 0
+#define F3D_SENSOR_DUCROS 1
 
 /* ---- context ------------------------------------------------------------ */
 /* Copies the parameter bundles to the device.  Replaces the Adapt-ed structs

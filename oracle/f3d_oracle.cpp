@@ -574,6 +574,35 @@ void or_shock_sensor(const or_dims* d, double* phi, const double* Q) {
       }
 }
 
+// Ducros sensor (no counterpart in the reference, which switches on the
+// pressure sensor above): theta = (div u)^2 / ((div u)^2 + |curl u|^2 + 1e-30)
+// from 2nd-order central differences of u, v, w along xi, eta, zeta and the
+// metrics (dxi_a/dx_j = M[3a + j] * J, the J^-1-weighted cofactors of
+// parse_mesh.jl:210-221), on the same points as shockSensor.
+void or_ducros_sensor(const or_dims* d, double* phi, const double* Q, const double* const* M,
+                      const double* J) {
+  Grid g(d);
+  const long sy = g.Nxt, sz = g.Nxt * g.Nyt;
+  const long st[3] = {1, sy, sz};
+#pragma omp parallel for schedule(static)
+  for (long k = 1; k < g.Nzt - 1; ++k)
+    for (long j = 1; j < g.Nyt - 1; ++j)
+      for (long i = 1; i < g.Nxt - 1; ++i) {
+        const long p = g.P(i, j, k);
+        double G[3][3];  // G[c][x] = d u_c / d x_x
+        for (int c = 0; c < 3; ++c) {
+          const double* u = Q + (1 + c) * g.NP;
+          double dxi[3];
+          for (int a = 0; a < 3; ++a) dxi[a] = 0.5 * (u[p + st[a]] - u[p - st[a]]);
+          for (int x = 0; x < 3; ++x)
+            G[c][x] = J[p] * (M[x][p] * dxi[0] + M[3 + x][p] * dxi[1] + M[6 + x][p] * dxi[2]);
+        }
+        const double dv = G[0][0] + G[1][1] + G[2][2];
+        const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+        phi[p] = dv * dv / (dv * dv + wx * wx + wy * wy + wz * wz + 1e-30);
+      }
+}
+
 static inline double minmod(double a, double b) {
   // schemes.jl:26-28
   return (a * b > 0) ? ((std::fabs(a) > std::fabs(b)) ? b : a) : 0.0;
@@ -1311,7 +1340,10 @@ void or_rk3_stage(const or_dims* d, int32_t KRK, double* U, double* Un, double* 
     std::memcpy(rhon, rhoi, sizeof(double) * NP * g.Ns);
   }
   or_mixture(d, Q, rhoi, Yi, lam, mu, D, th, tag);
-  or_shock_sensor(d, phi, Q);
+  if (sc->sensor_kind == 1)
+    or_ducros_sensor(d, phi, Q, M, J);
+  else
+    or_shock_sensor(d, phi, Q);
   // specAdvance
   or_species_split(d, rhoi, Q, Fpi, Fmi, M[0], M[1], M[2], tag);
   or_weno(d, 0, Fxi, phi, Fpi, Fmi, g.Ns, tag, sc);

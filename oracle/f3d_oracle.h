@@ -40,6 +40,8 @@ typedef struct {
   double SWeps2, WENOeps, phi1, phi2;
   double UP7[7];
   double T_criteria;
+  int32_t sensor_kind;  /* 0 the reference's pressure sensor, 1 Ducros */
+  int32_t pad;
 } or_scheme;
 
 /* ---- pointwise thermo / kinetics (thermo.jl, reactions.jl) ---- */
@@ -67,6 +69,8 @@ void or_species_split(const or_dims* d, const double* rhoi, const double* Q, dou
                       double* Fm, const double* Ax, const double* Ay, const double* Az,
                       const int64_t* tag);
 void or_shock_sensor(const or_dims* d, double* phi, const double* Q);
+void or_ducros_sensor(const or_dims* d, double* phi, const double* Q, const double* const* M,
+                      const double* J);
 void or_weno(const or_dims* d, int32_t dir, double* F, const double* phi, const double* Fp,
              const double* Fm, int32_t NV, const int64_t* tag, const or_scheme* sc);
 void or_viscous_flux(const or_dims* d, int32_t dir, double* Fv, const double* Q,

@@ -39,7 +39,8 @@ class _Kin(C.Structure):
 
 class _Scheme(C.Structure):
     _fields_ = [("SWeps2", C.c_double), ("WENOeps", C.c_double), ("phi1", C.c_double),
-                ("phi2", C.c_double), ("UP7", C.c_double * 7), ("T_criteria", C.c_double)]
+                ("phi2", C.c_double), ("UP7", C.c_double * 7), ("T_criteria", C.c_double),
+                ("sensor_kind", C.c_int32), ("pad", C.c_int32)]
 
 
 def build(force: bool = False) -> str:
@@ -94,7 +95,8 @@ class Oracle:
             s.update(scheme)
         self.scheme_dict = s
         self.scheme = _Scheme(s["SWeps2"], s["WENOeps"], s["phi1"], s["phi2"],
-                              (C.c_double * 7)(*s["UP7"]), s["T_criteria"])
+                              (C.c_double * 7)(*s["UP7"]), s["T_criteria"],
+                              int(s.get("sensor_kind", 0)), 0)
         L = self.lib
         L.or_get_T.restype = C.c_double
         L.or_get_T_from.restype = C.c_double
@@ -208,6 +210,11 @@ class Oracle:
     def shock_sensor(self, Q, phi=None):
         phi = self.zeros() if phi is None else phi
         self.lib.or_shock_sensor(C.byref(self.dims), _f(phi), _f(Q))
+        return phi
+
+    def ducros_sensor(self, Q, metrics, J, phi=None):
+        phi = self.zeros() if phi is None else phi
+        self.lib.or_ducros_sensor(C.byref(self.dims), _f(phi), _f(Q), self._mptr(metrics), _f(J))
         return phi
 
     def weno(self, d, phi, Fp, Fm, NV, tag=None):
