@@ -52,11 +52,12 @@ class Scheme(C.Structure):
     _fields_ = [("SWeps2", C.c_double), ("WENOeps", C.c_double), ("hybrid_phi1", C.c_double),
                 ("hybrid_phi2", C.c_double), ("UP7", C.c_double * 7), ("T_criteria", C.c_double),
                 ("precision", C.c_int32), ("bug_compat", C.c_int32), ("sensor_kind", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("recon_kind", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 NN_NPARAM = 12512  # F3D_NN_NPARAM
 SENSOR_PRESSURE, SENSOR_DUCROS = 0, 1  # F3D_SENSOR_*
+RECON_REFERENCE, RECON_WENO7 = 0, 1    # F3D_RECON_*
 # Ducros switch (UP7 below theta = 0.1, WENO5-Z above, never NND)
 DUCROS_SCHEME = dict(sensor_kind=SENSOR_DUCROS, hybrid_phi1=0.1, hybrid_phi2=2.0)
 
@@ -176,7 +177,7 @@ class Context:
         self.precision = precision
         self.rdtype = torch.float64 if precision == "fp64" else torch.float32
         s = dict(SWeps2=0.0, WENOeps=1e-26, hybrid_phi1=1e-3, hybrid_phi2=0.1, UP7=UP7,
-                 T_criteria=500.0, sensor_kind=SENSOR_PRESSURE)
+                 T_criteria=500.0, sensor_kind=SENSOR_PRESSURE, recon_kind=RECON_REFERENCE)
         if scheme:
             s.update(scheme)
         m = mech
@@ -193,7 +194,7 @@ class Context:
         sc = Scheme(s["SWeps2"], s["WENOeps"], s["hybrid_phi1"], s["hybrid_phi2"],
                     (C.c_double * 7)(*s["UP7"]), s["T_criteria"],
                     F3D_FP64 if precision == "fp64" else F3D_FP32, bug_compat,
-                    int(s["sensor_kind"]))
+                    int(s["sensor_kind"]), int(s["recon_kind"]))
         dims = Dims(Nx, Ny, Nz, NG, m.Ns, m.Nr)
         h = _vp()
         _rc(self.lib.f3d_create(C.byref(dims), C.byref(th), C.byref(kin), C.byref(sc),

@@ -74,6 +74,7 @@ static int fill_params(DevParams& P, const f3d_dims* d, const f3d_thermo* th,
   P.WENOeps = sc->WENOeps;
   P.phi1 = sc->hybrid_phi1;
   P.phi2 = sc->hybrid_phi2;
+  P.recon_kind = sc->recon_kind;
   for (int c = 0; c < 7; ++c) P.UP7[c] = sc->UP7[c];
   P.T_criteria = sc->T_criteria;
   // float copies
@@ -201,6 +202,10 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
   }
   if (sc->precision != F3D_FP64 && sc->precision != F3D_FP32) {
     set_error("f3d_create: precision must be F3D_FP64 or F3D_FP32");
+    return F3D_ERR_ARG;
+  }
+  if (sc->recon_kind != F3D_RECON_REFERENCE && sc->recon_kind != F3D_RECON_WENO7) {
+    set_error("f3d_create: recon_kind must be F3D_RECON_REFERENCE or F3D_RECON_WENO7");
     return F3D_ERR_ARG;
   }
   if (sc->sensor_kind != F3D_SENSOR_PRESSURE && sc->sensor_kind != F3D_SENSOR_DUCROS) {

@@ -29,6 +29,7 @@ struct DevParams {
   int vf[MAXNR][MAXNS], vr[MAXNR][MAXNS];
   double Arr[MAXNR][3], ef[MAXNR][MAXNS], loP[MAXNR][3], Troe[MAXNR][4];
   double SWeps2, WENOeps, phi1, phi2, UP7[7], T_criteria;
+  int recon_kind;  // 0 UP7 / WENO5-Z / NND (the reference), 1 WENO7-Z in the middle branch
   // float copies for the FP32 flux path
   float mw_f[MAXNS], sep_f[MAXNS], lo_f[MAXNS][7], hi_f[MAXNS][7];
   float visc_f[MAXNS][5], cond_f[MAXNS][5], bdiff_f[MAXNS][MAXNS][5];

@@ -174,6 +174,62 @@ __device__ inline void zweights(R s11, R s22, R s33, R e, R& w1, R& w2, R& w3) {
   }
 }
 
+// WENO7-Z face value from v[0..6] = v_{j-3..j+3} of the upwind cell j (no
+// counterpart in the reference, whose middle branch is WENO5-Z): Balsara &
+// Shu (2000) candidates and smoothness indicators, linear weights
+// (1, 12, 18, 4)/35, Z weights d_k (1 + (tau7/(eps + beta_k))^2) with
+// tau7 = |b0 + 3 b1 - 3 b2 - b3| (Castro, Costa & Don 2011); FP32 takes the
+// overflow-safe form of zweights.
+template <typename R>
+__device__ inline R weno7z(const R* v, R eps) {
+  // no FMA contraction: the smoothness indicators are long sums of products
+  // whose rounding the (tau / beta)^2 weights amplify near a jump; kept in the
+  // oracle's operation order so the FP64 path matches it to 1e-12
+#pragma clang fp contract(off)
+  const R a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6];
+  // indicators as quadratic forms in first differences (constants drop out
+  // exactly: no cancellation of the values' magnitude, which in float would
+  // swamp the indicators of a smooth stencil)
+  const R D0 = b - a, D1 = c - b, D2 = d - c, D3 = e - d, D4 = f - e, D5 = g - f;
+  const R r = R(1.0 / 240.0);
+  const R b0 = r * (D0 * (R(547) * D0 - R(2788) * D1 + R(1854) * D2) +
+                    D1 * (R(3708) * D1 - R(5188) * D2) + R(2107) * D2 * D2);
+  const R b1 = r * (D1 * (R(267) * D1 - R(1108) * D2 + R(494) * D3) +
+                    D2 * (R(1468) * D2 - R(1428) * D3) + R(547) * D3 * D3);
+  const R b2 = r * (D2 * (R(547) * D2 - R(1428) * D3 + R(494) * D4) +
+                    D3 * (R(1468) * D3 - R(1108) * D4) + R(267) * D4 * D4);
+  const R b3 = r * (D3 * (R(2107) * D3 - R(5188) * D4 + R(1854) * D5) +
+                    D4 * (R(3708) * D4 - R(2788) * D5) + R(547) * D5 * D5);
+  const R tau = fabs(b0 + R(3) * b1 - R(3) * b2 - b3);
+  R w0, w1, w2, w3;
+  if constexpr (sizeof(R) == 8) {
+    R t = tau / (eps + b0);
+    w0 = (R(1) + t * t) * R(1);
+    t = tau / (eps + b1);
+    w1 = (R(1) + t * t) * R(12);
+    t = tau / (eps + b2);
+    w2 = (R(1) + t * t) * R(18);
+    t = tau / (eps + b3);
+    w3 = (R(1) + t * t) * R(4);
+  } else {
+    const R q0 = eps + b0, q1 = eps + b1, q2 = eps + b2, q3 = eps + b3;
+    const R qm = fmin(fmin(q0, q1), fmin(q2, q3));
+    const R T = tau / qm;
+    const R u = R(1) / (R(1) + T * T);
+    const R vv = R(1) - u;
+    const R s0 = qm / q0, s1 = qm / q1, s2 = qm / q2, s3 = qm / q3;
+    w0 = (u + vv * (s0 * s0)) * R(1);
+    w1 = (u + vv * (s1 * s1)) * R(12);
+    w2 = (u + vv * (s2 * s2)) * R(18);
+    w3 = (u + vv * (s3 * s3)) * R(4);
+  }
+  const R c0 = -R(3) * a + R(13) * b - R(23) * c + R(25) * d;
+  const R c1 = b - R(5) * c + R(13) * d + R(3) * e;
+  const R c2 = -c + R(7) * d + R(7) * e - f;
+  const R c3 = R(3) * d + R(13) * e - R(5) * f + g;
+  return (w0 * c0 + w1 * c1 + w2 * c2 + w3 * c3) / (R(12) * (w0 + w1 + w2 + w3));
+}
+
 template <typename R>
 __device__ inline R minmod(R a, R b) {
   // schemes.jl:26-28
@@ -199,6 +255,10 @@ __device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams*
     const R fmv = c[6] * m[1] + c[5] * m[2] + c[4] * m[3] + c[3] * m[4] + c[2] * m[5] +
                   c[1] * m[6] + c[0] * m[7];
     return fpv + fmv;
+  } else if (br == 1 && P->recon_kind == 1) {
+    // WENO7-Z: F+ from the upwind cell ic-1 (p[0..6]), F- mirrored (m[7..1])
+    const R mr[7] = {m[7], m[6], m[5], m[4], m[3], m[2], m[1]};
+    return weno7z<R>(p, PT<R>::weps(P)) + weno7z<R>(mr, PT<R>::weps(P));
   } else if (br == 1) {
     const R tmp1 = R(13.0 / 12.0), tmp2 = R(1.0 / 6.0);
     const R e = PT<R>::weps(P);
@@ -252,7 +312,7 @@ __device__ inline R weno_face(const R* __restrict__ fp, const R* __restrict__ fm
                               R phif, const DevParams* __restrict__ P) {
   const int br = weno_branch<R>(phif, P);
   R p[8], m[8];
-  if (br == 0) {
+  if (br == 0 || (br == 1 && P->recon_kind == 1)) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) p[k] = fp[ic + (k - 4) * s];
 #pragma unroll

@@ -94,10 +94,15 @@ typedef struct {
                           from central differences and the metrics (needs f3d_set_metrics);
                           theta in [0, 1] then drives the same switch: hybrid_phi1 = 0.1,
                           hybrid_phi2 > 1 gives UP7 below 0.1 and WENO5-Z above */
-  int32_t reserved[5];
+  int32_t recon_kind;  /* F3D_RECON_REFERENCE: UP7 / WENO5-Z / NND (schemes.jl:86-437);
+                          F3D_RECON_WENO7: WENO7-Z (Balsara-Shu indicators, Z weights with
+                          tau7 = |b0 + 3 b1 - 3 b2 - b3|) in the middle branch */
+  int32_t reserved[4];
 } f3d_scheme;
 #define F3D_SENSOR_PRESSURE 0
 #define F3D_SENSOR_DUCROS 1
+#define F3D_RECON_REFERENCE 0
+#define F3D_RECON_WENO7 1
 
 /* ---- context ------------------------------------------------------------ */
 /* Copies the parameter bundles to the device.  Replaces the Adapt-ed structs

@@ -608,6 +608,44 @@ static inline double minmod(double a, double b) {
   return (a * b > 0) ? ((std::fabs(a) > std::fabs(b)) ? b : a) : 0.0;
 }
 
+// WENO7-Z reconstruction (no counterpart in the reference, whose middle
+// branch is WENO5-Z): the face value from the seven values v[0..6] =
+// v_{j-3..j+3} of the upwind cell j (Balsara & Shu 2000 candidates and
+// smoothness indicators, linear weights (1, 12, 18, 4)/35; Z weights
+// alpha_k = d_k (1 + (tau7 / (eps + beta_k))^2) with tau7 = |b0 + 3 b1 - 3 b2 - b3|,
+// Castro, Costa & Don 2011, the reference's WENO5-Z weight form).
+double or_weno7z(const double* v, double eps) {
+  const double a = v[0], b = v[1], c = v[2], d = v[3], e = v[4], f = v[5], g = v[6];
+  // the indicators as quadratic forms in the first differences of each
+  // candidate's four values (constants drop out exactly; no cancellation of
+  // the values' magnitude), the Balsara-Shu forms times 1/240
+  const double D0 = b - a, D1 = c - b, D2 = d - c, D3 = e - d, D4 = f - e, D5 = g - f;
+  const double r = 1.0 / 240.0;
+  const double b0 = r * (D0 * (547 * D0 - 2788 * D1 + 1854 * D2) + D1 * (3708 * D1 - 5188 * D2) +
+                         2107 * D2 * D2);
+  const double b1 = r * (D1 * (267 * D1 - 1108 * D2 + 494 * D3) + D2 * (1468 * D2 - 1428 * D3) +
+                         547 * D3 * D3);
+  const double b2 = r * (D2 * (547 * D2 - 1428 * D3 + 494 * D4) + D3 * (1468 * D3 - 1108 * D4) +
+                         267 * D4 * D4);
+  const double b3 = r * (D3 * (2107 * D3 - 5188 * D4 + 1854 * D5) + D4 * (3708 * D4 - 2788 * D5) +
+                         547 * D5 * D5);
+  const double tau = std::fabs(b0 + 3 * b1 - 3 * b2 - b3);
+  double t;
+  t = tau / (eps + b0);
+  const double w0 = (1 + t * t) * 1;
+  t = tau / (eps + b1);
+  const double w1 = (1 + t * t) * 12;
+  t = tau / (eps + b2);
+  const double w2 = (1 + t * t) * 18;
+  t = tau / (eps + b3);
+  const double w3 = (1 + t * t) * 4;
+  const double q0 = -3 * a + 13 * b - 23 * c + 25 * d;
+  const double q1 = b - 5 * c + 13 * d + 3 * e;
+  const double q2 = -c + 7 * d + 7 * e - f;
+  const double q3 = 3 * d + 13 * e - 5 * f + g;
+  return (w0 * q0 + w1 * q1 + w2 * q2 + w3 * q3) / (12 * (w0 + w1 + w2 + w3));
+}
+
 // WENO_x / WENO_y / WENO_z, schemes.jl:86-437.  V(o) = value at offset o
 // along the sweep from the face's right cell ic.
 static inline double weno_face(const double* fp, const double* fm, long ic, long s, double phif,
@@ -622,6 +660,14 @@ static inline double weno_face(const double* fp, const double* fm, long ic, long
     double fmv = c[6] * VM(-3) + c[5] * VM(-2) + c[4] * VM(-1) + c[3] * VM(0) + c[2] * VM(1) +
                  c[1] * VM(2) + c[0] * VM(3);
     return fpv + fmv;
+  } else if (phif < sc->phi2 && sc->recon_kind == 1) {
+    // upwind cell ic-1 for F+ (v_{j+m} = VP(m-1)), mirrored for F- (v_{j+m} = VM(-m))
+    double vp[7], vm[7];
+    for (int m = 0; m < 7; ++m) {
+      vp[m] = VP(m - 4);
+      vm[m] = VM(3 - m);
+    }
+    return or_weno7z(vp, sc->WENOeps) + or_weno7z(vm, sc->WENOeps);
   } else if (phif < sc->phi2) {
     const double e = sc->WENOeps;
     double V1 = VP(-3), V2 = VP(-2), V3 = VP(-1), V4 = VP(0), V5 = VP(1);

@@ -41,7 +41,7 @@ typedef struct {
   double UP7[7];
   double T_criteria;
   int32_t sensor_kind;  /* 0 the reference's pressure sensor, 1 Ducros */
-  int32_t pad;
+  int32_t recon_kind;   /* 0 the reference's UP7 / WENO5-Z / NND, 1 WENO7-Z in the middle branch */
 } or_scheme;
 
 /* ---- pointwise thermo / kinetics (thermo.jl, reactions.jl) ---- */
@@ -69,6 +69,7 @@ void or_species_split(const or_dims* d, const double* rhoi, const double* Q, dou
                       double* Fm, const double* Ax, const double* Ay, const double* Az,
                       const int64_t* tag);
 void or_shock_sensor(const or_dims* d, double* phi, const double* Q);
+double or_weno7z(const double* v7, double eps);
 void or_ducros_sensor(const or_dims* d, double* phi, const double* Q, const double* const* M,
                       const double* J);
 void or_weno(const or_dims* d, int32_t dir, double* F, const double* phi, const double* Fp,

@@ -40,7 +40,7 @@ class _Kin(C.Structure):
 class _Scheme(C.Structure):
     _fields_ = [("SWeps2", C.c_double), ("WENOeps", C.c_double), ("phi1", C.c_double),
                 ("phi2", C.c_double), ("UP7", C.c_double * 7), ("T_criteria", C.c_double),
-                ("sensor_kind", C.c_int32), ("pad", C.c_int32)]
+                ("sensor_kind", C.c_int32), ("recon_kind", C.c_int32)]
 
 
 def build(force: bool = False) -> str:
@@ -96,10 +96,11 @@ class Oracle:
         self.scheme_dict = s
         self.scheme = _Scheme(s["SWeps2"], s["WENOeps"], s["phi1"], s["phi2"],
                               (C.c_double * 7)(*s["UP7"]), s["T_criteria"],
-                              int(s.get("sensor_kind", 0)), 0)
+                              int(s.get("sensor_kind", 0)), int(s.get("recon_kind", 0)))
         L = self.lib
         L.or_get_T.restype = C.c_double
         L.or_get_T_from.restype = C.c_double
+        L.or_weno7z.restype = C.c_double
         L.or_internal_energy.restype = C.c_double
         L.or_cv.restype = C.c_double
         L.or_num_threads.restype = C.c_int32
@@ -211,6 +212,10 @@ class Oracle:
         phi = self.zeros() if phi is None else phi
         self.lib.or_shock_sensor(C.byref(self.dims), _f(phi), _f(Q))
         return phi
+
+    def weno7z(self, v7, eps=1e-26):
+        v = np.ascontiguousarray(v7, dtype=np.float64)
+        return self.lib.or_weno7z(_f(v), C.c_double(eps))
 
     def ducros_sensor(self, Q, metrics, J, phi=None):
         phi = self.zeros() if phi is None else phi
