@@ -621,8 +621,7 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   for (int d = 0; d < 3; ++d)
     RET(launch_spec_viscous(c, d, w[W_Fdx + d], Q, w[W_Yi], M, Jm, w[W_D], w[W_Fhx + d], tag, s,
                             mf));
-  RET(launch_div_specs(c, rhoi, w[W_Fxi], w[W_Fyi], w[W_Fzi], w[W_Fdx], w[W_Fdy], w[W_Fdz], dt,
-                       J, tag, s));
+  // (divSpecs is applied with div in launch_stage_end: flowAdvance does not read rho_k)
   // flowAdvance, solver.jl:22-38
   for (int d = 0; d < 3; ++d) {
     RET(launch_flux_split(c, Q, w[W_Fp], w[W_Fm], M[3 * d], M[3 * d + 1], M[3 * d + 2], tag, s,
@@ -632,15 +631,10 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   for (int d = 0; d < 3; ++d)
     RET(launch_viscous(c, d, w[W_Fvx + d], Q, M, Jm, w[W_lam], w[W_mu], w[W_Fhx + d], tag, s,
                        mf));
-  RET(launch_div(c, U, w[W_Fx], w[W_Fy], w[W_Fz], w[W_Fvx], w[W_Fvy], w[W_Fvz], dt, J, tag, s));
-  if (KRK == 2) {
-    RET(launch_lincomb(c, U, Un, 5, 0.25, 0.75, tag, s));
-    RET(launch_lincomb(c, rhoi, rhon, g.Ns, 0.25, 0.75, tag, s));
-  } else if (KRK == 3) {
-    RET(launch_lincomb(c, U, Un, 5, 2.0 / 3.0, 1.0 / 3.0, tag, s));
-    RET(launch_lincomb(c, rhoi, rhon, g.Ns, 2.0 / 3.0, 1.0 / 3.0, tag, s));
-  }
-  RET(launch_c2prim(c, U, Q, rhoi, tag, s));
+  // divSpecs + div + linComb (stages 2, 3) + c2Prim in one pass (k_stage_end)
+  void* const F[12] = {w[W_Fx],  w[W_Fy],  w[W_Fz],  w[W_Fvx], w[W_Fvy], w[W_Fvz],
+                       w[W_Fxi], w[W_Fyi], w[W_Fzi], w[W_Fdx], w[W_Fdy], w[W_Fdz]};
+  RET(launch_stage_end(c, KRK, U, Un, rhoi, rhon, Q, F, dt, J, tag, s));
   RET(launch_fill(c, Q, U, rhoi, s));
   return F3D_OK;
 }

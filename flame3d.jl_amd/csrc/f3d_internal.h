@@ -361,6 +361,9 @@ int launch_mixture(f3d_ctx* c, const double* Q, const double* rhoi, void* Yi, vo
                    void* mu, void* D, const int64_t* tag, hipStream_t s);
 int launch_c2prim(f3d_ctx* c, double* U, double* Q, double* rhoi, const int64_t* tag,
                   hipStream_t s);
+int launch_stage_end(f3d_ctx* c, int KRK, double* U, const double* Un, double* rhoi,
+                     const double* rhon, double* Q, void* const* F, double dt, const double* J,
+                     const int64_t* tag, hipStream_t s);
 int launch_prim2c(f3d_ctx* c, double* U, const double* Q, hipStream_t s);
 int launch_lincomb(f3d_ctx* c, double* U, const double* Un, int NV, double a, double b,
                    const int64_t* tag, hipStream_t s);

@@ -333,6 +333,129 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- launchers --------------------------------------------------------------
+// The end of an RK3 stage in one pass per interior cell: divSpecs and div
+// (div.jl:16-84: the face-flux differences into rho_k and U), for stages 2
+// and 3 linComb with the stage-1 copies (utils.jl:95-112, solver.jl:304-317),
+// then c2Prim (utils.jl:2-48) -- the arithmetic of k_div_specs, k_div,
+// k_lincomb and k_c2prim in the same order, with U and rho_k read once and
+// written once instead of three times (nothing between those kernels reads
+// a neighbouring cell, and flowAdvance does not read rho_k).
+template <int NS, typename R, bool WARM>
+__global__ void __launch_bounds__(kBlock)
+    k_stage_end(double* __restrict__ U, const double* __restrict__ Un, double* __restrict__ rhoi,
+                const double* __restrict__ rhon, double* __restrict__ Q, const R* __restrict__ Fx,
+                const R* __restrict__ Fy, const R* __restrict__ Fz, const R* __restrict__ Fvx,
+                const R* __restrict__ Fvy, const R* __restrict__ Fvz, const R* __restrict__ Fxi,
+                const R* __restrict__ Fyi, const R* __restrict__ Fzi, const R* __restrict__ Fdx,
+                const R* __restrict__ Fdy, const R* __restrict__ Fdz, double dt,
+                const double* __restrict__ J, const int64_t* __restrict__ tag,
+                const DevParams* __restrict__ P, Geom g, int lin, double la, double lb) {
+  const long id = gid();
+  if (id >= (long)g.Nx * g.Ny * g.Nz) return;
+  int a, b, c;
+  interior_idx(id, g, a, b, c);
+  const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  if (tag && tag[p] != 0) return;
+  const long NP = g.NP;
+  const long nfx = (long)(g.Nx + 1) * g.Ny * g.Nz, nfy = (long)g.Nx * (g.Ny + 1) * g.Nz,
+             nfz = (long)g.Nx * g.Ny * (g.Nz + 1);
+  const long fx = a + (long)(g.Nx + 1) * (b + (long)g.Ny * c);
+  const long fy = a + (long)g.Nx * (b + (long)(g.Ny + 1) * c);
+  const long fz = a + (long)g.Nx * (b + (long)g.Ny * c);
+  const long ox = 1, oy = g.Nx, oz = (long)g.Nx * g.Ny;
+  const R Jact = (R)(J[p] * dt);
+  // divSpecs
+  double r[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const long nx = n * nfx, ny = n * nfy, nz = n * nfz;
+    const R d = (Fxi[fx + nx] - Fxi[fx + ox + nx] + Fyi[fy + ny] - Fyi[fy + oy + ny] +
+                 Fzi[fz + nz] - Fzi[fz + oz + nz] + Fdx[fx + ox + nx] - Fdx[fx + nx] +
+                 Fdy[fy + oy + ny] - Fdy[fy + ny] + Fdz[fz + oz + nz] - Fdz[fz + nz]) *
+                Jact;
+    r[n] = rhoi[p + n * NP] + (double)d;
+  }
+  // div
+  R dV[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const R v1 = Fvx[fx + ox + q * nfx] - Fvx[fx + q * nfx];
+    const R v2 = Fvy[fy + oy + q * nfy] - Fvy[fy + q * nfy];
+    const R v3 = Fvz[fz + oz + q * nfz] - Fvz[fz + q * nfz];
+    dV[q] = v1 + v2 + v3;
+  }
+  double u[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const R d = (Fx[fx + n * nfx] - Fx[fx + ox + n * nfx] + Fy[fy + n * nfy] -
+                 Fy[fy + oy + n * nfy] + Fz[fz + n * nfz] - Fz[fz + oz + n * nfz]) *
+                Jact;
+    u[n] = U[p + n * NP] + (double)d;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) u[q + 1] += (double)(dV[q] * Jact);
+  // linComb
+  if (lin) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) u[n] = u[n] * la + Un[p + n * NP] * lb;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) r[n] = r[n] * la + rhon[p + n * NP] * lb;
+  }
+#pragma unroll
+  for (int n = 0; n < 5; ++n) U[p + n * NP] = u[n];
+  // c2Prim
+  const double rho = fmax(u[0], kEps);
+  const double rinv = 1 / rho;
+  double sr = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    r[n] = fmax(r[n], 0.0);
+    sr += r[n];
+  }
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r[n] *= rho / sr;
+  const double uu = u[1] * rinv;
+  const double vv = u[2] * rinv;
+  const double ww = u[3] * rinv;
+  const double ei = fmax(u[4] - 0.5 * rho * (uu * uu + vv * vv + ww * ww), kEps);
+  const double T = fmax(WARM ? get_T_from<NS>(P, ei, r, Q[p + 5 * NP]) : get_T<NS>(P, ei, r, nullptr),
+                       kEps);
+  const double pr = fmax(pmixture<NS>(P, T, r), kEps);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = r[n];
+  Q[p] = rho;
+  Q[p + NP] = uu;
+  Q[p + 2 * NP] = vv;
+  Q[p + 3 * NP] = ww;
+  Q[p + 4 * NP] = pr;
+  Q[p + 5 * NP] = T;
+  Q[p + 6 * NP] = ei;
+}
+
+int launch_stage_end(f3d_ctx* c, int KRK, double* U, const double* Un, double* rhoi,
+                     const double* rhon, double* Q, void* const* F, double dt, const double* J,
+                     const int64_t* tag, hipStream_t s) {
+  // F: Fx, Fy, Fz, Fvx, Fvy, Fvz, Fxi, Fyi, Fzi, Fdx, Fdy, Fdz
+  const long n = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+  const int lin = KRK > 1;
+  const double la = KRK == 2 ? 0.25 : 2.0 / 3.0, lb = KRK == 2 ? 0.75 : 1.0 / 3.0;
+#define F3D_SE(NS_, R_, W_)                                                                      \
+  hipLaunchKernelGGL((k_stage_end<NS_, R_, W_>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Un,   \
+                     rhoi, rhon, Q, (const R_*)F[0], (const R_*)F[1], (const R_*)F[2],            \
+                     (const R_*)F[3], (const R_*)F[4], (const R_*)F[5], (const R_*)F[6],          \
+                     (const R_*)F[7], (const R_*)F[8], (const R_*)F[9], (const R_*)F[10],         \
+                     (const R_*)F[11], dt, J, tag, c->dparams, c->g, lin, la, lb)
+  const bool f64 = c->precision == F3D_FP64;
+  switch (c->g.Ns) {
+    case 1: if (f64) F3D_SE(1, double, false); else F3D_SE(1, float, true); break;
+    case 5: if (f64) F3D_SE(5, double, false); else F3D_SE(5, float, true); break;
+    case 9: if (f64) F3D_SE(9, double, false); else F3D_SE(9, float, true); break;
+    default: set_error("stage_end: Ns must be 1, 5 or 9"); return F3D_ERR_UNSUPPORTED;
+  }
+#undef F3D_SE
+  return hip_check(hipGetLastError(), "stage_end");
+}
+
 int launch_div(f3d_ctx* c, double* U, const void* Fx, const void* Fy, const void* Fz,
                const void* Fvx, const void* Fvy, const void* Fvz, double dt, const double* J,
                const int64_t* tag, hipStream_t s) {
