@@ -92,6 +92,13 @@ static int fill_params(DevParams& P, const f3d_dims* d, const f3d_thermo* th,
     for (int nn = 0; nn < MAXNS; ++nn)
       for (int c = 0; c < 5; ++c) P.bdiff_f[n][nn][c] = (float)P.bdiff[n][nn][c];
   }
+  for (int n = 0; n < d->Ns; ++n)
+    for (int l = 0; l < d->Ns; ++l) {
+      const double wr = P.mw[l] / P.mw[n];
+      P.wr4_f[n][l] = (float)std::sqrt(std::sqrt(wr));
+      P.c8_f[n][l] = (float)(1.0 / std::sqrt(8.0 + 8.0 / wr));
+      P.iwr_f[n][l] = (float)(1.0 / wr);
+    }
   P.Ru_f = (float)P.Ru;
   P.WENOeps_f = (float)P.WENOeps;
   P.phi1_f = (float)P.phi1;

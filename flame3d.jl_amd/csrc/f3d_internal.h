@@ -34,6 +34,9 @@ struct DevParams {
   float mw_f[MAXNS], sep_f[MAXNS], lo_f[MAXNS][7], hi_f[MAXNS][7];
   float visc_f[MAXNS][5], cond_f[MAXNS][5], bdiff_f[MAXNS][MAXNS][5];
   float Ru_f, WENOeps_f, phi1_f, phi2_f, SWeps2_f, UP7_f[7];
+  // Wilke's species-pair constants for the FP32 mixture rule (W_l / W_n)^(1/4),
+  // 1 / sqrt(8 (1 + W_n / W_l)) and W_n / W_l
+  float wr4_f[MAXNS][MAXNS], c8_f[MAXNS][MAXNS], iwr_f[MAXNS][MAXNS];
 };
 
 // Padded-block geometry (0-based).  Point p = i + Nxt*(j + Nyt*k); variable

@@ -97,7 +97,7 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
   const R lgT2 = lgT * lgT;
   const R lgT3 = lgT * lgT2;
   const R lgT4 = lgT2 * lgT2;
-  R mui[NS];
+  R mui[NS], sqi[NS];
   R s1 = 0, s2 = 0;
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
@@ -112,23 +112,45 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
     const R* p = PT<R>::visc(P, n);
     const R sq = sqsqT * (p[0] + lgT * p[1] + lgT2 * p[2] + lgT3 * p[3] + lgT4 * p[4]);
     mui[n] = sq * sq;
+    sqi[n] = fabs(sq);
   }
   // Wilke, Phi[n][l] (thermo.jl:321-334)
   R phi[NS][NS];
+  if constexpr (sizeof(R) == 4) {
+    // FP32: sqrt(mu_n / mu_l) = |sq_n| / |sq_l| (mu = sq^2) and the species-pair
+    // factors precomputed, so no division or square root per pair
+    R isq[NS];
 #pragma unroll
-  for (int n = 0; n < NS; ++n) {
-    const R tmp1 = R(1) / PT<R>::mw(P, n);
-    const R tmp2 = mui[n];
+    for (int n = 0; n < NS; ++n) isq[n] = R(1) / sqi[n];
 #pragma unroll
-    for (int l = 0; l <= n; ++l) {
-      const R wr = PT<R>::mw(P, l) * tmp1;
-      const R vr = tmp2 / mui[l];
-      const R f1 = R(1) + sqrt(vr * sqrt(wr));
-      const R tmp = f1 * f1 / sqrt(R(8) + R(8) / wr);
-      phi[n][l] = tmp;
-      phi[l][n] = tmp / (vr * wr);
+    for (int n = 0; n < NS; ++n) {
+#pragma unroll
+      for (int l = 0; l < n; ++l) {
+        const R rs = sqi[n] * isq[l];  // sqrt(vr)
+        const R f1 = R(1) + rs * P->wr4_f[n][l];
+        const R tmp = f1 * f1 * P->c8_f[n][l];
+        phi[n][l] = tmp;
+        const R ir = sqi[l] * isq[n];  // 1 / sqrt(vr)
+        phi[l][n] = tmp * (ir * ir) * P->iwr_f[n][l];
+      }
+      phi[n][n] = R(1);
     }
-    phi[n][n] = R(1);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const R tmp1 = R(1) / PT<R>::mw(P, n);
+      const R tmp2 = mui[n];
+#pragma unroll
+      for (int l = 0; l <= n; ++l) {
+        const R wr = PT<R>::mw(P, l) * tmp1;
+        const R vr = tmp2 / mui[l];
+        const R f1 = R(1) + sqrt(vr * sqrt(wr));
+        const R tmp = f1 * f1 / sqrt(R(8) + R(8) / wr);
+        phi[n][l] = tmp;
+        phi[l][n] = tmp / (vr * wr);
+      }
+      phi[n][n] = R(1);
+    }
   }
   R m = 0;
 #pragma unroll
