@@ -397,6 +397,40 @@ def bench_air(a, torch, abi, cases, mm):
     c64.close()
 
 
+def bench_jet(a, torch, abi, cases, mm):
+    """The supersonic hot-air jet (5-species air, 5000 K, Ma 1.5 into 300 K
+    air) on a 2N x N x N stretched block, WENO7-Z in the middle branch of the
+    hybrid switch: FP32 stage + FP64 Newton air chemistry against FP64 stage +
+    the same chemistry.  The surrogate kernels are sized for the 9-species
+    H2/O2 network, so this workload runs its chemistry on the Newton path."""
+    mech = mm.load("air5")
+    N = (2 * a.size, a.size, a.size)
+    log(f"[bench] building the reacting jet case {N} on the GPU")
+    case = cases.reacting_jet(N, mech, dev="cuda")
+    nc = N[0] * N[1] * N[2]
+    stages = 3 * a.steps
+    scheme = {"recon_kind": abi.RECON_WENO7}
+    c64, r64 = run_path("fp64", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        scheme=scheme)
+    c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        scheme=scheme)
+    ms64, ms32 = r64["ms_total"] / stages, r32["ms_total"] / stages
+    errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
+    print(json.dumps({
+        "metric": "cell-updates/s per RK stage, FP32 flux vs FP64 (reacting air jet, Newton chemistry)",
+        "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 3 * ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 flux, f64 state and chemistry", "data": "synthetic",
+        "config": {"workload": f"hot air jet {N[0]}x{N[1]}x{N[2]}, 5-species air, 5000 K Ma 1.5 "
+                               "into 300 K, WENO7-Z/UP7/NND, Newton chemistry", "dt_s": case.dt},
+        "fp64": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
+                 "chem_ms_per_call": r64["chem_ms_mean"]},
+        "fp32": {"ms_per_stage": ms32, "chem_ms_per_call": r32["chem_ms_mean"]},
+        "speedup_fp32_vs_fp64": ms64 / ms32, "error_fp32_vs_fp64": errs}))
+    c32.close()
+    c64.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -409,8 +443,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--workload", choices=("reacting", "air"), default="reacting",
-                    help="air: the non-reacting 5-species air mixing layer, FP32 vs FP64 stage")
+    ap.add_argument("--workload", choices=("reacting", "air", "jet"), default="reacting",
+                    help="air: the non-reacting 5-species air mixing layer, FP32 vs FP64 stage; "
+                    "jet: the hot air jet at 2*size x size x size with Newton air chemistry")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="write the per-kernel HBM traffic summary from rocprofv3 --pmc runs")
     ap.add_argument("--pmc-out", metavar="FILE", help="where --pmc-summary writes (default: "
@@ -442,6 +477,8 @@ def main():
 
     if a.workload == "air":
         return bench_air(a, torch, abi, cases, mm)
+    if a.workload == "jet":
+        return bench_jet(a, torch, abi, cases, mm)
     mech = mm.load("h2_lidryer")
     N = (a.size, a.size, a.size)
     log(f"[bench] building the reacting mixing-layer case {N} on the GPU")

@@ -372,6 +372,42 @@ def reacting_mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", 
     return Case("reacting_mixing_layer", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
 
 
+def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0, D=0.01,
+                 Ma=1.5, T_jet=5000.0, T_air=300.0, P0=3596.0, noise=0.02, cfl=0.5) -> Case:
+    """Supersonic hot-air jet, the coupled reacting workload of 5-species air:
+    a round jet of undissociated air (O2 0.233, N2 0.767) at T_jet and Mach
+    Ma of its own sound speed along x through still air at T_air, both at
+    P0; the jet fills r < D/2 around the axis (y = 0, z = Lz/2) with a tanh
+    edge of thickness D/20, over the whole domain at t = 0 and held in the
+    x-min ghost cells (inflow); seeded white-noise velocity perturbations of
+    noise * U_jet inside the jet.  Grid Lx = 20 D, Ly = Lz = 10 D,
+    tanh-stretched in y around the axis (beta), uniform in x and z.  BCs:
+    x fixed inflow ghosts / zero-gradient outflow, y and z zero gradient."""
+    from . import abi
+    Nx, Ny, Nz = N
+    grid = stretched_y_grid(Nx, Ny, Nz, 20 * D, 10 * D, 10 * D, beta=beta, dev=dev)
+    r = torch.sqrt(grid.y ** 2 + (grid.z - 5 * D) ** 2)
+    th = 0.5 * (1 - torch.tanh((r - 0.5 * D) / (D / 20)))  # 1 inside the jet
+    Y = _Y_field(mech, AIR_Y, grid.x)
+    W = 1.0 / (0.233 / mech.mw[mech.index("O2")] + 0.767 / mech.mw[mech.index("N2")])
+    Rg = mech.Ru / W
+    Tj = torch.tensor([T_jet], dtype=torch.float64, device=dev)
+    cp = float(cp_mix(mech, Tj, _Y_field(mech, AIR_Y, Tj))[0])
+    gam = cp / (cp - Rg)
+    U_jet = Ma * math.sqrt(gam * Rg * T_jet)
+    T = T_air + (T_jet - T_air) * th
+    g = torch.Generator(device=dev).manual_seed(seed)
+    amp = noise * U_jet * th
+    u = U_jet * th + amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+    v = amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+    w = amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
+    P = torch.full_like(T, P0)
+    Q, rhoi = state_from_TPY(mech, T, P, Y, u, v, w)
+    dt = cfl_dt(grid, Q, cfl)
+    bc = (abi.BC_KEEP, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY)
+    return Case("reacting_jet", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
+
+
 def tgv_gas(Ma=0.1, Re=1600.0, T0=300.0, P0=101325.0, R_gas=287.0, gamma=1.4, Pr=0.72):
     """The calorically perfect gas of the Taylor-Green case (gamma 1.4, Pr 0.72)
     with the constant viscosity that gives Re = rho0 V0 L / mu at L = 1 m."""
