@@ -12,14 +12,17 @@ Reference pattern:
   * reactions.jl:2-56 (pre_input / post_predict) and solver.jl:204-253: the
     inference call, fused here into f3d_nn_chem with Newton fallback.
 
-Inputs (11): T, log P, BoxCox(Y_k) = (Y_k^lambda - 1)/lambda for the 9
-species; outputs (9): the increments dY_k over dt, learned as asinh(dY_k / s)
-with s = 1e-6 (linear below 1e-6, where dY moves T by < 0.1 K, logarithmic
-above, so the network's relative accuracy holds over the decades that matter
-for T); the reference's Box-Cox-space increments and plain dY are the other
-label forms.  Inputs and labels are z-scored; the Mahalanobis gate uses the
-mean and inverse covariance of the z-scored training inputs and the training
-distances' 3-sigma quantile.
+Inputs (11): T, log P, BoxCox(max(Y_k, 1e-12)) = (Y^lambda - 1)/lambda for
+the 9 species; outputs (9): the increments dY_k over dt, learned as
+asinh(dY_k / s_k) with the per-cell scale s_k = min(1e-6, max(1e-14,
+0.1 Y_k)) (label_mode 3): linear below s_k and logarithmic above, so the
+network's relative accuracy holds over the decades that matter for T, and a
+trace species' increment stays relative to the trace (a fixed s = 1e-6 left
+an absolute error floor of ~1e-7 that seeded radicals into the free streams
+of a flow); the reference's Box-Cox-space increments, plain dY and the fixed
+scale are the other label forms.  Inputs and labels are z-scored; the
+Mahalanobis gate uses the mean and inverse covariance of the z-scored
+training inputs and the training distances' 3-sigma quantile.
 """
 from __future__ import annotations
 

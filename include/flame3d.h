@@ -231,7 +231,8 @@ typedef struct {
                             relative to Y_k for trace species, so a species at or
                             near zero stays there (the element projection is then
                             weighted per cell by (s_k + |dY_k|)^2) */
-  float label_scale[9];  /* label_mode 2: dY_k = label_scale_k sinh(out_k) */
+  float label_scale[9];  /* label_mode 2: dY_k = label_scale_k sinh(out_k); mode 3: the cap
+                            of s_k */
   float label_rel;       /* label_mode 3 (see above) */
   float label_floor;     /* label_mode 3 (see above) */
   float y_floor;         /* inputs use BoxCox(max(Y_k, y_floor)); 0: BoxCox(Y_k) */
