@@ -392,16 +392,17 @@ class Context:
 
     def _nn_batch(self, X, Y, ny):
         t = self.torch
-        if X.dim() != 2 or X.shape[1] != 11:
-            raise F3DError(f"X: expected (B, 11), got {tuple(X.shape)}")
+        nin = self.Ns + 2
+        if X.dim() != 2 or X.shape[1] != nin:
+            raise F3DError(f"X: expected (B, {nin}), got {tuple(X.shape)}")
         B = X.shape[0]
-        self._chk(X, (B, 11), t.float32, "X")
+        self._chk(X, (B, nin), t.float32, "X")
         self._chk(Y, (B, ny), t.float32, "Y")
         return B
 
     def nn_train_step(self, X, Y, lr, loss=None):
         """One Adam step; loss: optional device float32 tensor of one element."""
-        B = self._nn_batch(X, Y, 9)
+        B = self._nn_batch(X, Y, self.Ns)
         lp = None
         if loss is not None:
             lp = self._chk(loss, (1,), self.torch.float32, "loss")
@@ -410,15 +411,15 @@ class Context:
 
     def nn_grad(self, X, Y):
         """Gradient (NN_NPARAM floats) and loss of the batch, as a device tensor."""
-        B = self._nn_batch(X, Y, 9)
+        B = self._nn_batch(X, Y, self.Ns)
         g = self.torch.empty(NN_NPARAM + 1, dtype=self.torch.float32, device=X.device)
         _rc(self.lib.f3d_nn_grad(self.h, X.data_ptr(), Y.data_ptr(), B, g.data_ptr(),
                                  self.stream()), "f3d_nn_grad")
         return g
 
     def nn_forward(self, X):
-        Y = self.torch.empty((X.shape[0], 9), dtype=self.torch.float32, device=X.device)
-        B = self._nn_batch(X, Y, 9)
+        Y = self.torch.empty((X.shape[0], self.Ns), dtype=self.torch.float32, device=X.device)
+        B = self._nn_batch(X, Y, self.Ns)
         _rc(self.lib.f3d_nn_forward(self.h, X.data_ptr(), Y.data_ptr(), B, self.stream()),
             "f3d_nn_forward")
         return Y

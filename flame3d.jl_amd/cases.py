@@ -373,7 +373,8 @@ def reacting_mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", 
 
 
 def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0, D=0.01,
-                 Ma=1.5, T_jet=5000.0, T_air=300.0, P0=3596.0, noise=0.02, cfl=0.5) -> Case:
+                 Ma=1.5, T_jet=5000.0, T_air=300.0, P0=3596.0, noise=0.02, cfl=0.5,
+                 dt=None) -> Case:
     """Supersonic hot-air jet, the coupled reacting workload of 5-species air:
     a round jet of undissociated air (O2 0.233, N2 0.767) at T_jet and Mach
     Ma of its own sound speed along x through still air at T_air, both at
@@ -382,7 +383,10 @@ def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0,
     x-min ghost cells (inflow); seeded white-noise velocity perturbations of
     noise * U_jet inside the jet.  Grid Lx = 20 D, Ly = Lz = 10 D,
     tanh-stretched in y around the axis (beta), uniform in x and z.  BCs:
-    x fixed inflow ghosts / zero-gradient outflow, y and z zero gradient."""
+    x fixed inflow ghosts / zero-gradient outflow, y and z zero gradient.
+    The time step is the CFL step unless `dt` fixes it (a surrogate trained
+    for one chemistry step needs a fixed one; it must not exceed the CFL
+    step)."""
     from . import abi
     Nx, Ny, Nz = N
     grid = stretched_y_grid(Nx, Ny, Nz, 20 * D, 10 * D, 10 * D, beta=beta, dev=dev)
@@ -403,7 +407,11 @@ def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0,
     w = amp * torch.randn(T.shape, generator=g, device=dev, dtype=torch.float64)
     P = torch.full_like(T, P0)
     Q, rhoi = state_from_TPY(mech, T, P, Y, u, v, w)
-    dt = cfl_dt(grid, Q, cfl)
+    dt_cfl = cfl_dt(grid, Q, cfl)
+    if dt is None:
+        dt = dt_cfl
+    elif dt > dt_cfl:
+        raise ValueError(f"reacting_jet: dt {dt:g} exceeds the CFL step {dt_cfl:g}")
     bc = (abi.BC_KEEP, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY, abi.BC_COPY)
     return Case("reacting_jet", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
 

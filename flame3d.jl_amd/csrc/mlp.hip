@@ -71,8 +71,8 @@ __device__ inline float sum_q(float v) {
 }
 
 __global__ void __launch_bounds__(256)
-    k_nn_train_tile(const float* __restrict__ Xg, const float* __restrict__ Tg, int B,
-                    const __bf16* __restrict__ Wb, const __bf16* __restrict__ Wtb,
+    k_nn_train_tile(const float* __restrict__ Xg, const float* __restrict__ Tg, int B, int nin,
+                    int nout, const __bf16* __restrict__ Wb, const __bf16* __restrict__ Wtb,
                     const float* __restrict__ master, float* __restrict__ partial) {
   __shared__ __align__(16) TileLds L;
   const int tid = threadIdx.x;
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256)
   const long s0 = (long)blockIdx.x * 64;
   for (int e = tid; e < 64 * NN_INP; e += 256) {
     const int s = e / NN_INP, k = e % NN_INP;
-    const float v = (k < NN_IN) ? Xg[(s0 + s) * NN_IN + k] : 0.0f;
+    const float v = (k < nin) ? Xg[(s0 + s) * nin + k] : 0.0f;
     L.X[s][k] = (__bf16)v;
     L.Xt[k][s] = (__bf16)v;
   }
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();  // A[0] (layer-2 output) fully read before dZ overwrites the union
   {
     const float* bias = master + nn_b_off(3);
-    const float scale = 2.0f / ((float)B * (float)NN_OUT);
+    const float scale = 2.0f / ((float)B * (float)nout);
 #pragma unroll
     for (int ot = 0; ot < NN_OUTP / 16; ++ot) {
       const int o = 16 * ot + r16;
@@ -144,8 +144,8 @@ __global__ void __launch_bounds__(256)
       for (int i = 0; i < 4; ++i) {
         const int s = sw + 4 * q + i;
         float g = 0.0f;
-        if (o < NN_OUT) {
-          const float d = yv[ot][i] + bias[o] - Tg[(s0 + s) * NN_OUT + o];
+        if (o < nout) {
+          const float d = yv[ot][i] + bias[o] - Tg[(s0 + s) * nout + o];
           lsum += d * d;
           g = scale * d;
         }
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256)
     if (tid < st) L.red[tid] += L.red[tid + st];
     __syncthreads();
   }
-  if (tid == 0) part[NN_NPARAM] = L.red[0] / ((float)B * (float)NN_OUT);
+  if (tid == 0) part[NN_NPARAM] = L.red[0] / ((float)B * (float)nout);
 }
 
 // Sum of the per-tile partials (gradient and loss), one thread per slot.
@@ -334,17 +334,17 @@ __device__ __forceinline__ void nn_forward_wave(FwdLds& L, const __bf16* __restr
   nn_layer_wave<NN_H, NN_OUTP, true>(L, Wb + nn_w_off(3), master + nn_b_off(3), lane);
 }
 
-// plain forward on normalized inputs X[B][NN_IN] -> Y[B][NN_OUT] (B % 64 == 0)
+// plain forward on normalized inputs X[B][nin] -> Y[B][nout] (B % 64 == 0)
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-    k_nn_forward(const float* __restrict__ Xg, float* __restrict__ Yg, const __bf16* __restrict__ Wb,
-                 const float* __restrict__ master) {
+    k_nn_forward(const float* __restrict__ Xg, float* __restrict__ Yg, int nin, int nout,
+                 const __bf16* __restrict__ Wb, const float* __restrict__ master) {
   __shared__ __align__(16) FwdLds L;
   const int lane = threadIdx.x;
   const long s0 = (long)blockIdx.x * 64;
-  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? Xg[(s0 + lane) * NN_IN + k] : 0.0f);
+  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < nin ? Xg[(s0 + lane) * nin + k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
-  for (int o = 0; o < NN_OUT; ++o) Yg[(s0 + lane) * NN_OUT + o] = L.Y[lane][o];
+  for (int o = 0; o < nout; ++o) Yg[(s0 + lane) * nout + o] = L.Y[lane][o];
 }
 
 // Element-conserving projection with per-cell weights S:
@@ -453,13 +453,14 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
 //      carriers below chain_max (near dt lambda = 1 a hot radical-free mixture
 //      has a near-root with a tiny residual while the Newton step ignites it).
 // ---------------------------------------------------------------------------
+template <int NS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_nn_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
               const int64_t* __restrict__ tag, const DevParams* __restrict__ P,
               const NNDev* __restrict__ C, const __bf16* __restrict__ Wb,
               const float* __restrict__ master, Geom g, double dt,
               int32_t* __restrict__ tile_count, int8_t* __restrict__ status) {
-  constexpr int NS = NN_OUT;
+  constexpr int NIN = NS + 2;  // T, log P, Box-Cox of each species
   __shared__ __align__(16) FwdLds L;
   const int lane = threadIdx.x;
   const long ncell = (long)g.Nx * g.Ny * g.Nz;
@@ -469,9 +470,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   long p = 0;
   bool act = false;
   double r[NS], rho = 0.0, T = 0.0;
-  float xn[NN_IN];
+  float xn[NIN];
 #pragma unroll
-  for (int k = 0; k < NN_IN; ++k) xn[k] = 0.0f;
+  for (int k = 0; k < NIN; ++k) xn[k] = 0.0f;
   if (id < ncell) {
     const int a = (int)(id % g.Nx);
     const int t = (int)(id / g.Nx);
@@ -486,7 +487,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     rho = Q[p];
     const double rinv = 1.0 / fmax(rho, kEps);
     const float lam = C->boxcox_lambda;
-    float x[NN_IN];
+    float x[NIN];
     x[0] = (float)T;
     x[1] = logf((float)Q[p + 4 * NP]);
 #pragma unroll
@@ -496,10 +497,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
     }
 #pragma unroll
-    for (int k = 0; k < NN_IN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
+    for (int k = 0; k < NIN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
   }
 #pragma unroll
-  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NN_IN ? xn[k] : 0.0f);
+  for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NIN ? xn[k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
   int code = act ? 1 : 0;  // 1 accepted, 2..8 the first gate that failed
@@ -507,10 +508,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     // 1. Mahalanobis distance of the normalized inputs
     float d2 = 0.0f;
 #pragma unroll
-    for (int i = 0; i < NN_IN; ++i) {
+    for (int i = 0; i < NIN; ++i) {
       float row = 0.0f;
 #pragma unroll
-      for (int j = 0; j < NN_IN; ++j) row += C->mahal_inv[i][j] * (xn[j] - C->mahal_mean[j]);
+      for (int j = 0; j < NIN; ++j) row += C->mahal_inv[i][j] * (xn[j] - C->mahal_mean[j]);
       d2 += (xn[i] - C->mahal_mean[i]) * row;
     }
     if (!(sqrtf(fmaxf(d2, 0.0f)) <= C->d_max)) code = 2;
@@ -716,19 +717,20 @@ __global__ void __launch_bounds__(64)
 // singular.
 static int nn_projection(const f3d_ctx* c, const f3d_nn_config* cfg, double (&Pm)[NN_OUT][NN_OUT]) {
   const int ne = cfg->n_elem;
+  const int ns = c->g.Ns;  // species of the context (<= NN_OUT)
   for (int i = 0; i < NN_OUT; ++i)
     for (int j = 0; j < NN_OUT; ++j) Pm[i][j] = (i == j) ? 1.0 : 0.0;
   if (ne == 0) return 0;
   double E[4][NN_OUT], S[NN_OUT], G[4][8];
   for (int e = 0; e < ne; ++e)
-    for (int k = 0; k < NN_OUT; ++k) E[e][k] = cfg->elem[e][k] / c->host_params.mw[k];
-  for (int k = 0; k < NN_OUT; ++k) S[k] = (double)cfg->out_std[k] * cfg->out_std[k];
+    for (int k = 0; k < ns; ++k) E[e][k] = cfg->elem[e][k] / c->host_params.mw[k];
+  for (int k = 0; k < ns; ++k) S[k] = (double)cfg->out_std[k] * cfg->out_std[k];
   // G = [E S E^T | I], Gauss-Jordan with partial pivoting -> inverse in G[:, ne:]
   for (int a = 0; a < ne; ++a)
     for (int b = 0; b < 2 * ne; ++b) {
       double v = 0.0;
       if (b < ne)
-        for (int k = 0; k < NN_OUT; ++k) v += E[a][k] * S[k] * E[b][k];
+        for (int k = 0; k < ns; ++k) v += E[a][k] * S[k] * E[b][k];
       else
         v = (b - ne == a) ? 1.0 : 0.0;
       G[a][b] = v;
@@ -750,8 +752,8 @@ static int nn_projection(const f3d_ctx* c, const f3d_nn_config* cfg, double (&Pm
       }
   }
   // Pm -= S E^T Ginv E
-  for (int i = 0; i < NN_OUT; ++i)
-    for (int j = 0; j < NN_OUT; ++j) {
+  for (int i = 0; i < ns; ++i)
+    for (int j = 0; j < ns; ++j) {
       double v = 0.0;
       for (int a = 0; a < ne; ++a)
         for (int b = 0; b < ne; ++b) v += S[i] * E[a][i] * G[a][ne + b] * E[b][j];
@@ -800,10 +802,11 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
     set_error("f3d_nn_init: null argument");
     return F3D_ERR_ARG;
   }
-  if (c->g.Ns != NN_OUT) {
-    set_error("f3d_nn_init: the surrogate predicts 9 species (context Ns must be 9)");
+  if (c->g.Ns != 9 && c->g.Ns != 5) {
+    set_error("f3d_nn_init: the surrogate step is built for Ns = 9 (H2/O2) or 5 (air)");
     return F3D_ERR_UNSUPPORTED;
   }
+  const int ns = c->g.Ns, nin = c->g.Ns + 2;  // outputs and inputs used (<= 9, <= 11)
   if (cfg->n_elem < 0 || cfg->n_elem > 4 || !(cfg->dt > 0) || !(cfg->boxcox_lambda > 0) ||
       cfg->label_mode < 0 || cfg->label_mode > 3 || !(cfg->y_floor >= 0)) {
     set_error("f3d_nn_init: need 0 <= n_elem <= 4, dt > 0, boxcox_lambda > 0, label_mode 0..3, "
@@ -812,13 +815,13 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   }
   if (cfg->label_mode == 3) {
     bool okl = cfg->label_rel > 0 && cfg->label_floor > 0;
-    for (int k = 0; k < NN_OUT; ++k) okl = okl && cfg->label_scale[k] >= cfg->label_floor;
+    for (int k = 0; k < ns; ++k) okl = okl && cfg->label_scale[k] >= cfg->label_floor;
     if (!okl) {
       set_error("f3d_nn_init: label_mode 3 needs label_rel > 0 and label_scale >= label_floor > 0");
       return F3D_ERR_ARG;
     }
   }
-  for (int k = 0; k < NN_IN; ++k)
+  for (int k = 0; k < nin; ++k)
     if (!(cfg->in_std[k] > 0)) {
       set_error("f3d_nn_init: in_std must be positive");
       return F3D_ERR_ARG;
@@ -846,7 +849,7 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   {
     int nc = 0;
     for (int j = 0; j < 3; ++j) h.chain_idx[j] = -1;
-    for (int k = 0; k < NN_OUT; ++k)
+    for (int k = 0; k < ns; ++k)
       if ((cfg->chain_species >> k) & 1) {
         if (nc == 3) {
           set_error("f3d_nn_init: chain_species names more than 3 species");
@@ -854,8 +857,8 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
         }
         h.chain_idx[nc++] = k;
       }
-    if ((cfg->chain_species >> NN_OUT) != 0 || (cfg->chain_max > 0 && nc == 0)) {
-      set_error("f3d_nn_init: chain_species must name 1..3 of the 9 species when chain_max > 0");
+    if ((cfg->chain_species >> ns) != 0 || (cfg->chain_max > 0 && nc == 0)) {
+      set_error("f3d_nn_init: chain_species must name 1..3 of the context's species when chain_max > 0");
       return F3D_ERR_ARG;
     }
   }
@@ -954,7 +957,7 @@ int f3d_nn_grad(f3d_ctx* c, const float* X, const float* Y, int32_t B, float* gr
   const long tiles = B / 64;
   rc = nn_partials(c, tiles);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B,
+  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B, c->g.Ns + 2, c->g.Ns,
                      (const __bf16*)c->nn_Wb, (const __bf16*)c->nn_Wtb, c->nn_master,
                      c->nn_partial);
   hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(NN_NPARAM + 1)), dim3(kBlock), 0, s,
@@ -973,7 +976,7 @@ int f3d_nn_train_step(f3d_ctx* c, const float* X, const float* Y, int32_t B, flo
   const double b1 = 0.9, b2 = 0.999;
   const float bc1 = (float)(1.0 - std::pow(b1, (double)c->nn_step));
   const float bc2 = (float)(1.0 - std::pow(b2, (double)c->nn_step));
-  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B,
+  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B, c->g.Ns + 2, c->g.Ns,
                      (const __bf16*)c->nn_Wb, (const __bf16*)c->nn_Wtb, c->nn_master,
                      c->nn_partial);
   hipLaunchKernelGGL(k_nn_adam, dim3(nblocks(NN_NPARAM + 1)), dim3(kBlock), 0, s, c->nn_partial,
@@ -985,8 +988,8 @@ int f3d_nn_train_step(f3d_ctx* c, const float* X, const float* Y, int32_t B, flo
 int f3d_nn_forward(f3d_ctx* c, const float* X, float* Y, int32_t B, hipStream_t s) {
   int rc = nn_batch_args(c, X, Y, B, "f3d_nn_forward");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_nn_forward, dim3(B / 64), dim3(64), 0, s, X, Y, (const __bf16*)c->nn_Wb,
-                     c->nn_master);
+  hipLaunchKernelGGL(k_nn_forward, dim3(B / 64), dim3(64), 0, s, X, Y, c->g.Ns + 2, c->g.Ns,
+                     (const __bf16*)c->nn_Wb, c->nn_master);
   return hip_check(hipGetLastError(), "f3d_nn_forward");
 }
 
@@ -1011,9 +1014,14 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   const long ntile = (ncell + 63) / 64;
   int8_t* st = status ? status : c->nn_status;
   // every tile writes its count, so no clearing is needed
-  hipLaunchKernelGGL(k_nn_chem, dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
-                     c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
-                     c->g, dt, c->nn_fbcount, st);
+  if (c->g.Ns == 9)
+    hipLaunchKernelGGL((k_nn_chem<9>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
+                       c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
+                       c->g, dt, c->nn_fbcount, st);
+  else
+    hipLaunchKernelGGL((k_nn_chem<5>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
+                       c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
+                       c->g, dt, c->nn_fbcount, st);
   hipLaunchKernelGGL(k_nn_scan, dim3(1), dim3(1024), 0, s, c->nn_fbcount, ntile, c->nn_fbtotal);
   hipLaunchKernelGGL(k_nn_compact, dim3((unsigned)ntile), dim3(64), 0, s, st, c->nn_fbcount,
                      ncell, c->nn_fblist);

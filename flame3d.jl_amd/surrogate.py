@@ -38,7 +38,7 @@ import numpy as np
 from . import abi, cases
 
 NG = abi.NG
-NN_IN, NN_OUT, NPARAM = 11, 9, abi.NN_NPARAM
+NN_IN, NN_OUT, NPARAM = 11, 9, abi.NN_NPARAM  # largest network (H2/O2); air uses 7 / 5
 LAMBDA = 0.1   # NN/H2/gen_data.py: lamda = 0.1
 DT = 1e-7      # NN/H2/gen_data.py: dt = 1e-7
 SEED = 12345   # lux-ck.jl:35-36
@@ -62,12 +62,15 @@ def _offsets():
 W_OFF, B_OFF = _offsets()
 
 
-def init_params(seed: int = SEED) -> np.ndarray:
-    """Glorot-uniform weights on the true fan-in/fan-out, zero biases and
-    zero padding (Lux.Dense's default initialization)."""
+def init_params(seed: int = SEED, nin: int = NN_IN, nout: int = NN_OUT) -> np.ndarray:
+    """Glorot-uniform weights on the true fan-in/fan-out (nin = Ns + 2 inputs,
+    nout = Ns outputs), zero biases and zero padding (Lux.Dense's default
+    initialization)."""
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, np.float32)
     for l, (ni, no, fi, fo) in enumerate(_LAYERS):
+        fi = nin if l == 0 else fi
+        fo = nout if l == len(_LAYERS) - 1 else fo
         a = math.sqrt(6.0 / (fi + fo))
         W = np.zeros((no, ni), np.float32)
         W[:fo, :fi] = rng.uniform(-a, a, size=(fo, fi))
@@ -116,7 +119,8 @@ class Dataset:
         if label_mode == 0:
             return self.Y1 - self.Y0
         if label_mode in (2, 3):
-            sc = torch.as_tensor(label_scale, dtype=self.Y0.dtype, device=self.Y0.device)
+            sc = torch.as_tensor(list(label_scale)[:self.Y0.shape[1]], dtype=self.Y0.dtype,
+                                 device=self.Y0.device)
             if label_mode == 3:
                 sc = torch.minimum(sc, (label_rel * self.Y0).clamp_min(label_floor))
             return torch.asinh((self.Y1 - self.Y0) / sc)
@@ -126,7 +130,7 @@ class Dataset:
 def generate(mech, shape=(64, 64, 32), steps: int = 2000, n_record: int = 32, seed: int = SEED,
              maxiter: int = 50, rtol: float = 1e-10, ttol: float = 1e-6,
              T_range=(900.0, 2500.0), P_atm=(1.0, 10.0), z_pure: float = 0.1,
-             trace: float = 0.0, log=print) -> Dataset:
+             trace: float = 0.0, dt: float = DT, log=print) -> Dataset:
     """Trajectory samples (NN/H2/gen_data.py pattern): Nx*Ny*Nz reactors drawn
     from the H2/O2/N2 distribution (T ~ U[900, 2500] K, P ~ U[1, 10] atm,
     mixture fraction ~ Beta(2, 2), plus a fraction z_pure each of pure
@@ -135,12 +139,18 @@ def generate(mech, shape=(64, 64, 32), steps: int = 2000, n_record: int = 32, se
     steps (the first four, then random) every reactor contributes the pair
     (state, state after one step).  With trace > 0 every species of every
     reactor is seeded with a trace, log-uniform in [1e-14, trace] (the states
-    that diffusion leaves in the free streams of a flow)."""
+    that diffusion leaves in the free streams of a flow).  A mechanism
+    without H2 (5-species air) draws the dissociating-air reactors of
+    cases.chem_cells_air over T_range and P_atm instead."""
     import torch
     Nx, Ny, Nz = shape
     ctx = abi.Context(Nx, Ny, Nz, mech, precision="fp64")
-    Q, rhoi = cases.chem_cells(shape, mech, seed=seed, dev="cuda", T_range=T_range, P_atm=P_atm,
-                               z_pure=z_pure)
+    if "H2" in mech.species:
+        Q, rhoi = cases.chem_cells(shape, mech, seed=seed, dev="cuda", T_range=T_range,
+                                   P_atm=P_atm, z_pure=z_pure)
+    else:
+        Q, rhoi = cases.chem_cells_air(shape, mech, seed=seed, T_range=T_range, P_atm=P_atm)
+        Q, rhoi = Q.cuda().contiguous(), rhoi.cuda().contiguous()
     if trace > 0:
         g = torch.Generator(device="cuda").manual_seed(seed + 1)
         rho = Q[0]
@@ -165,13 +175,13 @@ def generate(mech, shape=(64, 64, 32), steps: int = 2000, n_record: int = 32, se
         if step in rec:
             X = features(Q, rhoi)
             Y0 = mass_fractions(Q, rhoi)
-            ctx.chem_newton(U, Q, rhoi, DT, maxiter, rtol, ttol, iters=iters)
+            ctx.chem_newton(U, Q, rhoi, dt, maxiter, rtol, ttol, iters=iters)
             Xs.append(X)
             Y0s.append(Y0)
             Y1s.append(mass_fractions(Q, rhoi))
             it_sum += float(iters.double().mean())
         else:
-            ctx.chem_newton(U, Q, rhoi, DT, maxiter, rtol, ttol)
+            ctx.chem_newton(U, Q, rhoi, dt, maxiter, rtol, ttol)
         if log and (step + 1) % 500 == 0:
             torch.cuda.synchronize()
             log(f"  generate: step {step + 1}/{steps}, {len(Xs)} records, {time.time() - t0:.1f}s")
@@ -181,16 +191,24 @@ def generate(mech, shape=(64, 64, 32), steps: int = 2000, n_record: int = 32, se
 
 def generate_flow(mech, N=(192, 192, 192), steps: int = 8, per_call: int = 262144,
                   seed: int = 999, maxiter: int = 50, rtol: float = 1e-10, ttol: float = 1e-6,
-                  log=print) -> Dataset:
-    """Samples from the reacting H2/air mixing layer itself (a different seed
-    and resolution from the benchmark's): the FP64 flow with Strang Newton
-    chemistry is run for `steps` time steps, and at every chemistry half-step
-    `per_call` random interior cells contribute (state, state after the
-    backward-Euler step of DT).  This covers the states a flow produces and
-    0-D reactors do not (e.g. fresh gas with radicals diffused into it)."""
+                  dt: float = DT, log=print) -> Dataset:
+    """Samples from a reacting flow itself (a different seed and resolution
+    from the benchmark's): the reacting H2/air mixing layer, or for 5-species
+    air the hot supersonic jet on a 2N x N x N block.  The FP64 flow with
+    Strang Newton chemistry (half-steps of dt) is run for `steps` time steps,
+    and at every chemistry half-step `per_call` random interior cells
+    contribute (state, state after the backward-Euler step of dt).  This
+    covers the states a flow produces and 0-D reactors do not (e.g. fresh gas
+    with radicals diffused into it)."""
     import torch
-    case = cases.reacting_mixing_layer(N, mech, seed=seed, dev="cuda", dt=2 * DT)
-    ctx = abi.Context(*N, mech, precision="fp64")
+    if "H2" in mech.species:
+        case = cases.reacting_mixing_layer(N, mech, seed=seed, dev="cuda", dt=2 * dt)
+        scheme = None
+    else:
+        N = (2 * N[0], N[1], N[2])
+        case = cases.reacting_jet(N, mech, seed=seed, dev="cuda", dt=2 * dt)
+        scheme = {"recon_kind": abi.RECON_WENO7}
+    ctx = abi.Context(*N, mech, precision="fp64", scheme=scheme)
     ctx.set_metrics(case.grid.metrics, case.grid.J)
     ctx.set_boundary(case.bc)
     Q, r = case.Q.clone(), case.rhoi.clone()
@@ -205,7 +223,7 @@ def generate_flow(mech, N=(192, 192, 192), steps: int = 8, per_call: int = 26214
         idx = torch.randint(0, ncell, (per_call,), generator=g, device="cuda")
         Xs.append(features(Q, r)[idx])
         Y0s.append(mass_fractions(Q, r)[idx])
-        ctx.chem_newton(U, Q, r, DT, maxiter, rtol, ttol)
+        ctx.chem_newton(U, Q, r, dt, maxiter, rtol, ttol)
         Y1s.append(mass_fractions(Q, r)[idx])
         ctx.fill(Q, U, r)
 
@@ -256,13 +274,14 @@ def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = L
                 chain_max: float = CHAIN_MAX, chain_species=CHAIN_SPECIES,
                 label_rel: float = LABEL_REL, label_floor: float = LABEL_FLOOR,
                 y_floor: float = Y_FLOOR, sigmas: float = 3.0,
-                ridge: float = 1e-6) -> abi.NNConfig:
+                ridge: float = 1e-6, dt: float = DT) -> abi.NNConfig:
     """Normalization and gate statistics from the training data.  The
     Mahalanobis threshold is the training distances' quantile at the one-sided
     `sigmas`-sigma coverage of a normal law (0.99865 for 3 sigma)."""
     import torch
     X = floor_inputs(data.X, y_floor)
-    lab = data.labels(label_mode, [label_scale] * NN_OUT, label_rel, label_floor)
+    nin, nout = X.shape[1], mech.Ns  # Ns + 2 inputs, Ns outputs
+    lab = data.labels(label_mode, [label_scale] * nout, label_rel, label_floor)
     if not (bool(torch.isfinite(X).all()) and bool(torch.isfinite(lab).all())):
         raise ValueError("training data hold non-finite values")
     in_mean = X.mean(0)
@@ -272,20 +291,20 @@ def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = L
     Xn = (X - in_mean) / in_std
     mu = Xn.mean(0)
     Xc = Xn - mu
-    cov = (Xc.T @ Xc / (Xn.shape[0] - 1)).cpu().numpy() + ridge * np.eye(NN_IN)
+    cov = (Xc.T @ Xc / (Xn.shape[0] - 1)).cpu().numpy() + ridge * np.eye(nin)
     inv_np = np.linalg.inv(cov)
     inv = torch.from_numpy(inv_np).to(X.device)
     d = torch.sqrt(torch.clamp(((Xc @ inv) * Xc).sum(1), min=0.0))
     cover = 0.5 * (1.0 + math.erf(sigmas / math.sqrt(2.0)))
     d_max = float(torch.quantile(d[:: max(1, d.numel() // 1_000_000)].float(), cover))
     cfg = abi.NNConfig()
-    for k in range(NN_IN):
+    for k in range(nin):
         cfg.in_mean[k] = float(in_mean[k])
         cfg.in_std[k] = float(in_std[k])
         cfg.mahal_mean[k] = float(mu[k])
-        for j in range(NN_IN):
-            cfg.mahal_inv[k * NN_IN + j] = float(inv[k, j])
-    for k in range(NN_OUT):
+        for j in range(nin):
+            cfg.mahal_inv[k * NN_IN + j] = float(inv[k, j])  # row stride 11 in every case
+    for k in range(nout):
         cfg.out_mean[k] = float(out_mean[k])
         cfg.out_std[k] = float(out_std[k])
     cfg.d_max = d_max
@@ -296,10 +315,12 @@ def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = L
     cfg.dT_max = dT_max
     cfg.chi_max = chi_max
     cfg.chain_max = chain_max
-    cfg.chain_species = sum(1 << mech.index(n) for n in chain_species)
+    cfg.chain_species = sum(1 << mech.index(n) for n in chain_species if n in mech.species)
+    if cfg.chain_species == 0:
+        cfg.chain_max = 0.0  # no chain carriers in this mechanism (air): gate off
     cfg.boxcox_lambda = LAMBDA
     cfg.label_mode = label_mode
-    for k in range(NN_OUT):
+    for k in range(nout):
         cfg.label_scale[k] = label_scale
     cfg.label_rel = label_rel
     cfg.label_floor = label_floor
@@ -307,9 +328,9 @@ def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = L
     names, comp = _elements(mech)
     cfg.n_elem = len(names)
     for e in range(len(names)):
-        for k in range(NN_OUT):
+        for k in range(nout):
             cfg.elem[e][k] = float(comp[e, k])
-    cfg.dt = DT
+    cfg.dt = dt
     return cfg
 
 
@@ -317,10 +338,11 @@ def normalize(cfg: abi.NNConfig, data: Dataset):
     """z-scored float32 inputs and labels (label form from cfg.label_mode)."""
     import torch
     dev = data.X.device
-    im = torch.tensor(list(cfg.in_mean), dtype=torch.float64, device=dev)
-    isd = torch.tensor(list(cfg.in_std), dtype=torch.float64, device=dev)
-    om = torch.tensor(list(cfg.out_mean), dtype=torch.float64, device=dev)
-    osd = torch.tensor(list(cfg.out_std), dtype=torch.float64, device=dev)
+    nin, nout = data.X.shape[1], data.Y0.shape[1]
+    im = torch.tensor(list(cfg.in_mean)[:nin], dtype=torch.float64, device=dev)
+    isd = torch.tensor(list(cfg.in_std)[:nin], dtype=torch.float64, device=dev)
+    om = torch.tensor(list(cfg.out_mean)[:nout], dtype=torch.float64, device=dev)
+    osd = torch.tensor(list(cfg.out_std)[:nout], dtype=torch.float64, device=dev)
     Xn = ((floor_inputs(data.X, cfg.y_floor) - im) / isd).float().contiguous()
     lab = data.labels(cfg.label_mode, list(cfg.label_scale), cfg.label_rel, cfg.label_floor)
     Yn = ((lab - om) / osd).float().contiguous()
@@ -399,7 +421,8 @@ def load(path: str = DEFAULT_PATH):
 
 
 def attach(ctx: abi.Context, path: str = DEFAULT_PATH, **gate_overrides):
-    """Load a trained surrogate into a context (Ns = 9)."""
+    """Load a trained surrogate into a context of the mechanism it was
+    trained for (9-species H2/O2 or 5-species air)."""
     cfg, params, meta = load(path)
     for k, v in gate_overrides.items():
         setattr(cfg, k, v)

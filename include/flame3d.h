@@ -193,7 +193,10 @@ int f3d_chem_newton(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt,
  * Reference pattern: pre_input / evalModel / post_predict, reactions.jl:2-72
  * (Box-Cox inputs, z-scored inputs and labels, GEMM + GELU layers).  Inputs
  * are (T, log P, BoxCox(Y_1..Y_9)), outputs dY_1..dY_9 over the step dt; the
- * network is trained here, on the GPU, by f3d_nn_train_step. */
+ * network is trained here, on the GPU, by f3d_nn_train_step.  Contexts with
+ * Ns = 9 (H2/O2: 11 inputs, 9 outputs) or Ns = 5 (air: 7 inputs, 5 outputs);
+ * the configuration arrays are sized for the larger and use their first
+ * Ns + 2 / Ns entries (mahal_inv keeps its row stride of 11). */
 #define F3D_NN_NPARAM 12512 /* packed FP32 parameters, layout in csrc/f3d_nn.h */
 typedef struct {
   float in_mean[11], in_std[11], out_mean[9], out_std[9];
@@ -250,7 +253,7 @@ int f3d_nn_init(f3d_ctx* ctx, const f3d_nn_config* cfg);
 int f3d_nn_set_params(f3d_ctx* ctx, const float* params);
 int f3d_nn_get_params(f3d_ctx* ctx, float* params);
 /* One Adam step on the MSE of normalized outputs for a device batch
- * X[B][11] (normalized inputs), Y[B][9] (normalized targets), B % 64 == 0;
+ * X[B][Ns+2] (normalized inputs), Y[B][Ns] (normalized targets), B % 64 == 0;
  * loss (device float, may be NULL) receives sum (y - t)^2 / (B * 9). */
 int f3d_nn_train_step(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float lr,
                       float* loss, hipStream_t s);
@@ -258,7 +261,7 @@ int f3d_nn_train_step(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, f
  * floats, followed by the loss) without updating the weights. */
 int f3d_nn_grad(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float* grad,
                 hipStream_t s);
-/* Forward on normalized inputs: Y[B][9] = net(X[B][11]), B % 64 == 0. */
+/* Forward on normalized inputs: Y[B][Ns] = net(X[B][Ns+2]), B % 64 == 0. */
 int f3d_nn_forward(f3d_ctx* ctx, const float* X, float* Y, int32_t B, hipStream_t s);
 /* Surrogate chemistry step of dt (= cfg.dt) over the interior: cells with
  * T > T_criteria are predicted by the network; cells failing the gates are
