@@ -260,7 +260,7 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
         def chem(h):
             pass
     elif surrogate is not None:
-        surrogate.attach(ctx)
+        surrogate(ctx)
 
         def chem(h):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -314,6 +314,18 @@ def field_errors(torch, a, b, names):
                      "linf": float(d.abs().max()) / scale,
                      "linf_abs": float(d.abs().max())}
     return out
+
+
+def state_errors(ra, rb, mech):
+    """Mass-fraction and temperature differences of two runs' interiors."""
+    sl = (slice(None),) + (slice(4, -4),) * 3
+    Ya = ra["rhoi"][sl] / ra["Q"][0][sl[1:]].unsqueeze(0)
+    Yb = rb["rhoi"][sl] / rb["Q"][0][sl[1:]].unsqueeze(0)
+    dY = (Ya - Yb).abs()
+    dT = (ra["Q"][5][sl[1:]] - rb["Q"][5][sl[1:]]).abs()
+    return {"Y_linf": float(dY.max()), "Y_l2": float(dY.pow(2).mean().sqrt()),
+            "T_linf_K": float(dT.max()), "T_l2_K": float(dT.pow(2).mean().sqrt()),
+            "per_species_Y_linf": {sp: float(dY[k].max()) for k, sp in enumerate(mech.species)}}
 
 
 def cpu_baseline(mech, size, steps, min_seconds=10.0):
@@ -400,35 +412,55 @@ def bench_air(a, torch, abi, cases, mm):
 def bench_jet(a, torch, abi, cases, mm):
     """The supersonic hot-air jet (5-species air, 5000 K, Ma 1.5 into 300 K
     air) on a 2N x N x N stretched block, WENO7-Z in the middle branch of the
-    hybrid switch: FP32 stage + FP64 Newton air chemistry against FP64 stage +
-    the same chemistry.  The surrogate kernels are sized for the 9-species
-    H2/O2 network, so this workload runs its chemistry on the Newton path."""
+    hybrid switch, at the fixed dt of 1.2e-8 s the air surrogate was trained
+    for (chemistry half-steps of 6e-9 s): FP64 stage + FP64 Newton air
+    chemistry (the plain method), FP32 stage + the same Newton chemistry, and
+    FP32 stage + the BF16 MLP air chemistry with Newton fallback."""
+    import flame3d_amd.surrogate as sur
     mech = mm.load("air5")
     N = (2 * a.size, a.size, a.size)
     log(f"[bench] building the reacting jet case {N} on the GPU")
-    case = cases.reacting_jet(N, mech, dev="cuda")
+    case = cases.reacting_jet(N, mech, dev="cuda", dt=2 * sur.AIR_DT)
     nc = N[0] * N[1] * N[2]
     stages = 3 * a.steps
     scheme = {"recon_kind": abi.RECON_WENO7}
     c64, r64 = run_path("fp64", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
                         scheme=scheme)
+    c64.close()
+    ms64 = r64["ms_total"] / stages
+    log(f"[bench] FP64 + Newton {ms64:.3f} ms/stage (chemistry {r64['chem_ms_mean']:.2f} ms/call)")
     c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
                         scheme=scheme)
-    ms64, ms32 = r64["ms_total"] / stages, r32["ms_total"] / stages
-    errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
-    print(json.dumps({
-        "metric": "cell-updates/s per RK stage, FP32 flux vs FP64 (reacting air jet, Newton chemistry)",
-        "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 3 * ms32, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 flux, f64 state and chemistry", "data": "synthetic",
-        "config": {"workload": f"hot air jet {N[0]}x{N[1]}x{N[2]}, 5-species air, 5000 K Ma 1.5 "
-                               "into 300 K, WENO7-Z/UP7/NND, Newton chemistry", "dt_s": case.dt},
-        "fp64": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
-                 "chem_ms_per_call": r64["chem_ms_mean"]},
-        "fp32": {"ms_per_stage": ms32, "chem_ms_per_call": r32["chem_ms_mean"]},
-        "speedup_fp32_vs_fp64": ms64 / ms32, "error_fp32_vs_fp64": errs}))
     c32.close()
-    c64.close()
+    ms32 = r32["ms_total"] / stages
+    log(f"[bench] FP32 + Newton {ms32:.3f} ms/stage (chemistry {r32['chem_ms_mean']:.2f} ms/call)")
+    cnn, rnn = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
+                        scheme=scheme, surrogate=lambda c: sur.attach(c, sur.AIR_PATH))
+    cnn.close()
+    msnn = rnn["ms_total"] / stages
+    nfb = rnn["fallback"]
+    fb = sum(nfb) / (nc * len(nfb))
+    log(f"[bench] FP32 + NN {msnn:.3f} ms/stage (chemistry {rnn['chem_ms_mean']:.2f} ms/call, "
+        f"fallback {fb:.4f})")
+    names = ["rho", "u", "v", "w", "p", "T", "e"]
+    print(json.dumps({
+        "metric": "cell-updates/s per RK stage, FP32+NN vs FP64 plain (reacting air jet)",
+        "value": nc / (msnn * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 3 * msnn, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 flux + bf16 MLP + f64 chemistry/state",
+        "data": "synthetic",
+        "config": {"workload": f"hot air jet {N[0]}x{N[1]}x{N[2]}, 5-species air, 5000 K Ma 1.5 "
+                               "into 300 K, WENO7-Z/UP7/NND, Strang chemistry", "dt_s": case.dt},
+        "fp64_newton": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
+                        "chem_ms_per_call": r64["chem_ms_mean"]},
+        "fp32_newton": {"value": nc / (ms32 * 1e-3), "ms_per_stage": ms32,
+                        "chem_ms_per_call": r32["chem_ms_mean"],
+                        "error_vs_fp64": state_errors(r32, r64, mech)},
+        "fp32_nn": {"ms_per_stage": msnn, "chem_ms_per_call": rnn["chem_ms_mean"],
+                    "fallback_fraction": fb},
+        "speedup_vs_fp64_plain": ms64 / msnn, "speedup_fp32_newton_vs_fp64": ms64 / ms32,
+        "error_vs_fp64_plain": state_errors(rnn, r64, mech),
+        "error_fields_vs_fp64_plain": field_errors(torch, rnn["Q"], r64["Q"], names)}))
 
 
 def main():
@@ -491,19 +523,12 @@ def main():
     log(f"[bench] FP64 plain {ms64:.3f} ms/stage (chemistry {r64['chem_ms_mean']:.2f} ms/call); "
         "FP32 + surrogate path")
     c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
-                        surrogate=sur)
+                        surrogate=sur.attach)
     ms32 = r32["ms_total"] / stages
     log(f"[bench] FP32+NN {ms32:.3f} ms/stage (chemistry {r32['chem_ms_mean']:.2f} ms/call, "
         f"fallback {sum(r32['fallback']) / (nc * len(r32['fallback'])):.3f})")
     errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
-    sl = (slice(None),) + (slice(4, -4),) * 3
-    Y32 = r32["rhoi"][sl] / r32["Q"][0][sl[1:]].unsqueeze(0)
-    Y64 = r64["rhoi"][sl] / r64["Q"][0][sl[1:]].unsqueeze(0)
-    dY = (Y32 - Y64).abs()
-    dT = (r32["Q"][5][sl[1:]] - r64["Q"][5][sl[1:]]).abs()
-    acc = {"Y_linf": float(dY.max()), "Y_l2": float(dY.pow(2).mean().sqrt()),
-           "T_linf_K": float(dT.max()), "T_l2_K": float(dT.pow(2).mean().sqrt()),
-           "per_species_Y_linf": {sp: float(dY[k].max()) for k, sp in enumerate(mech.species)}}
+    acc = state_errors(r32, r64, mech)
     value = nc / (ms32 * 1e-3)
 
     # per-kernel roofline (FP32 path); dominant = largest time per stage

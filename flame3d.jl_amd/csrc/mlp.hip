@@ -482,6 +482,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     act = (!tag || tag[p] == 0) && (T > P->T_criteria);
     status[id] = 0;
   }
+  // a tile with no reacting cell (cold gas, tagged cells) skips the network;
+  // the workgroup is one wave, so the exit is uniform over the workgroup
+  if (!__any(act)) {
+    if (lane == 0) tile_count[tile] = 0;
+    return;
+  }
   if (act) {
     // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
     rho = Q[p];

@@ -44,6 +44,8 @@ DT = 1e-7      # NN/H2/gen_data.py: dt = 1e-7
 SEED = 12345   # lux-ck.jl:35-36
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "mechanisms", "h2_surrogate.npz")
+AIR_PATH = os.path.join(HERE, "mechanisms", "air5_surrogate.npz")
+AIR_DT = 6e-9  # the air network's chemistry step: the jet's half-step at its fixed dt of 1.2e-8 s
 
 # layer shapes of the packed parameter vector (layout in csrc/f3d_nn.h)
 _LAYERS = [(32, 64, 11, 64), (64, 64, 64, 64), (64, 64, 64, 64), (64, 32, 64, 9)]
@@ -316,8 +318,8 @@ def make_config(mech, data: Dataset, label_mode: int = 3, label_scale: float = L
     cfg.chi_max = chi_max
     cfg.chain_max = chain_max
     cfg.chain_species = sum(1 << mech.index(n) for n in chain_species if n in mech.species)
-    if cfg.chain_species == 0:
-        cfg.chain_max = 0.0  # no chain carriers in this mechanism (air): gate off
+    if cfg.chain_species == 0:  # carriers absent from the mechanism are dropped (air keeps O)
+        cfg.chain_max = 0.0
     cfg.boxcox_lambda = LAMBDA
     cfg.label_mode = label_mode
     for k in range(nout):

@@ -44,7 +44,6 @@ def _load_package():
     return mod
 
 
-AIR_DT = 6e-9               # jet chemistry half-step at its fixed dt of 1.2e-8 s
 AIR_T, AIR_P = (1500.0, 6000.0), (0.02, 0.1)
 H2_T, H2_P = (900.0, 2500.0), (1.0, 10.0)
 
@@ -55,7 +54,7 @@ def evaluate(pkg, mech, path, shape=(64, 64, 64), seed=777, log=print, dt=None,
     import torch
     abi, cases, sur = pkg.abi, pkg.cases, pkg.surrogate
     air = "H2" not in mech.species
-    dt = dt or (AIR_DT if air else sur.DT)
+    dt = dt or (sur.AIR_DT if air else sur.DT)
     Nx, Ny, Nz = shape
     ctx = abi.Context(Nx, Ny, Nz, mech, precision="fp64")
     sur.attach(ctx, path, **gates)
@@ -141,7 +140,7 @@ def main(argv=None):
 
     mech = mm.load(a.mech)
     air = a.mech == "air5"
-    dt = a.dt or (AIR_DT if air else sur.DT)
+    dt = a.dt or (sur.AIR_DT if air else sur.DT)
     T_range = tuple(a.T_range or (AIR_T if air else H2_T))
     P_atm = tuple(a.P_atm or (AIR_P if air else H2_P))
     t0 = time.time()

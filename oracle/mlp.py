@@ -11,7 +11,9 @@ packed FP32 vector of 12512 floats, inputs padded to 32 and outputs to 32
 
     W1 [64][32] | b1 [64] | W2 [64][64] | b2 [64] | W3 [64][64] | b3 [64] | W4 [32][64] | b4 [32]
 
-Loss L = sum_{s, o < 9} (y_so - t_so)^2 / (B * 9).
+Loss L = sum_{s, o < nout} (y_so - t_so)^2 / (B * nout), nout = 9 for H2/O2
+(11 inputs) and 5 for 5-species air (7 inputs, the rest of the 11 input
+slots zero).
 
 `emulate_bf16=True` rounds (round to nearest even) exactly the values the GPU
 kernels hold in BF16: the inputs, the weights, every hidden activation and
@@ -89,11 +91,12 @@ def gelu_grad(x):
 
 
 def forward(params, X, emulate_bf16=False):
-    """y [B, 9] and the cache (activations, pre-activations) for backward."""
+    """y [B, 9] and the cache (activations, pre-activations) for backward;
+    X [B, nin] with nin <= 11 (missing inputs are zero)."""
     r = bf16 if emulate_bf16 else (lambda v: np.asarray(v, dtype=np.float64))
     B = X.shape[0]
     A = np.zeros((B, NN_INP))
-    A[:, :NN_IN] = X
+    A[:, :X.shape[1]] = X
     A = r(A)
     acts, pre = [A], []
     layers = unpack(params)
@@ -107,14 +110,14 @@ def forward(params, X, emulate_bf16=False):
 
 
 def loss_grad(params, X, T, emulate_bf16=False):
-    """(loss, grad[NPARAM]) of L = sum (y - t)^2 / (B * 9)."""
+    """(loss, grad[NPARAM]) of L = sum (y - t)^2 / (B * nout), T [B, nout]."""
     r = bf16 if emulate_bf16 else (lambda v: np.asarray(v, dtype=np.float64))
-    B = X.shape[0]
+    B, nout = T.shape
     y, (acts, pre) = forward(params, X, emulate_bf16)
-    d = y - T
-    loss = float(np.sum(d * d) / (B * NN_OUT))
+    d = y[:, :nout] - T
+    loss = float(np.sum(d * d) / (B * nout))
     g = np.zeros((B, NN_OUTP))
-    g[:, :NN_OUT] = 2.0 * d / (B * NN_OUT)
+    g[:, :nout] = 2.0 * d / (B * nout)
     layers = unpack(params)
     grad = np.zeros(NPARAM)
     for l in range(3, -1, -1):

@@ -10,7 +10,9 @@
     the backward-Euler residual gate agrees with the oracle's residual;
   * the trained H2 surrogate on fresh reactors: species and temperature error
     against the Newton step, the fallback fraction, and the Newton work it
-    removes (what the network gains over the integrator alone).
+    removes (what the network gains over the integrator alone);
+  * the same kernels on a 5-species air context (7 inputs, 5 outputs), and
+    the trained air surrogate on fresh dissociating-air reactors.
 """
 import os
 
@@ -110,6 +112,56 @@ def test_gradient_matches_oracle():
     # padded weights get exactly zero gradient
     pad = [mlp.W_OFF[0] + o * 32 + k for o in range(64) for k in range(11, 32)]
     pad += list(range(mlp.W_OFF[3] + 9 * 64, mlp.B_OFF[3])) + list(range(mlp.B_OFF[3] + 9, 12512))
+    assert np.all(g[pad] == 0)
+    ctx.close()
+
+
+def _air_ctx(size=(8, 8, 8)):
+    import flame3d_amd.mech as mm
+    abi = _abi()
+    ctx = abi.Context(*size, mm.load("air5"), "fp64")
+    cfg = abi.NNConfig()
+    for k in range(11):
+        cfg.in_std[k] = 1.0
+    for k in range(9):
+        cfg.out_std[k] = 1.0
+    cfg.d_max = 1e30
+    cfg.elem_tol = 1e30
+    cfg.neg_tol = 1.0
+    cfg.boxcox_lambda = 0.1
+    cfg.dt = 6e-9
+    ctx.nn_init(cfg)
+    return ctx, cfg
+
+
+def test_air_forward_and_gradient_match_oracle():
+    """Ns = 5: the kernels read 7 inputs per sample and write / train on 5
+    outputs; the oracle sees the same 7 inputs (other input slots zero) and
+    the loss over the first 5 outputs.  Bars as for H2/O2."""
+    print("\n[surrogate] air forward B=256, gradient B=512", flush=True)
+    ctx, _ = _air_ctx()
+    p = _rand_params(11)
+    ctx.nn_set_params(p)
+    rng = np.random.default_rng(12)
+    X = rng.normal(0, 1, (256, 7)).astype(np.float32)
+    y = ctx.nn_forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    assert y.shape == (256, 5)
+    ye = mlp.forward(p.astype(np.float64), X.astype(np.float64), emulate_bf16=True)[0][:, :5]
+    assert _rel(y, ye) < 5e-3, _rel(y, ye)
+    X = rng.normal(0, 1, (512, 7)).astype(np.float32)
+    T = rng.normal(0, 1, (512, 5)).astype(np.float32)
+    g = ctx.nn_grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()).cpu().numpy()
+    le, ge = mlp.loss_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64),
+                           emulate_bf16=True)
+    assert abs(g[-1] - le) / le < 1e-3, (g[-1], le)
+    for l, (ni, no) in enumerate(mlp.LAYERS):
+        w = slice(mlp.W_OFF[l], mlp.W_OFF[l] + ni * no)
+        b = slice(mlp.B_OFF[l], mlp.B_OFF[l] + no)
+        assert _rel(g[w], ge[w]) < 2e-2, (l, "W", _rel(g[w], ge[w]))
+        assert _rel(g[b], ge[b]) < 2e-2, (l, "b", _rel(g[b], ge[b]))
+    # unused input columns (7..31) and output rows (5..31) get exactly zero gradient
+    pad = [mlp.W_OFF[0] + o * 32 + k for o in range(64) for k in range(7, 32)]
+    pad += list(range(mlp.W_OFF[3] + 5 * 64, mlp.B_OFF[3])) + list(range(mlp.B_OFF[3] + 5, 12512))
     assert np.all(g[pad] == 0)
     ctx.close()
 
@@ -480,6 +532,67 @@ def test_trained_surrogate_accuracy_and_gain():
     assert frac <= 0.5
     assert avoided >= 0.5
     assert times["surrogate"] < times["newton"]
+    ctx.close()
+
+
+AIR_SURROGATE = os.path.join(os.path.dirname(SURROGATE), "air5_surrogate.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(AIR_SURROGATE), reason="trained air surrogate not present")
+def test_trained_air_surrogate_accuracy():
+    """Fresh dissociating-air reactors (seed unseen in training; T ~ U[1500,
+    6000] K, P ~ U[0.02, 0.1] atm) at 64^3, one chemistry step of 6e-9 s:
+    surrogate with gates and Newton fallback vs the Newton step alone.
+    Bars: species error <= 1e-3 and temperature error <= 1 K (Linf), the
+    network takes over at least half of the cells."""
+    import flame3d_amd.cases as cases
+    import flame3d_amd.mech as mm
+    import flame3d_amd.surrogate as sur
+    abi = _abi()
+    shape = (64, 64, 64)
+    print(f"\n[surrogate] trained air network on {shape} fresh reactors", flush=True)
+    m = mm.load("air5")
+    ctx = abi.Context(*shape, m, "fp64")
+    cfg, _ = sur.attach(ctx, AIR_SURROGATE)
+    # trained for the 6e-9 s step; of the chain carriers only O is in air
+    assert abs(cfg.dt - sur.AIR_DT) < 1e-20 and cfg.chain_species == 1 << m.index("O")
+    Q, r = cases.chem_cells_air(shape, m, seed=2024, T_range=(1500.0, 6000.0),
+                                P_atm=(0.02, 0.1))
+    Q, r = Q.cuda().contiguous(), r.cuda().contiguous()
+    U = torch.zeros((5,) + tuple(Q.shape[1:]), dtype=torch.float64, device="cuda")
+    Qn, rn, Un = Q.clone(), r.clone(), U.clone()
+    Q0, r0 = Q.clone(), r.clone()
+    ctx.chem_newton(Un, Qn, rn, sur.AIR_DT, 50, 1e-10, 1e-6)
+    st = torch.zeros((shape[2], shape[1], shape[0]), dtype=torch.int8, device="cuda")
+    nfb = ctx.nn_chem(U, Q, r, sur.AIR_DT, 50, 1e-10, 1e-6, status=st)
+    torch.cuda.synchronize()
+    Ye = (sur.mass_fractions(Q, r) - sur.mass_fractions(Qn, rn)).abs().max().item()
+    Te = (sur.interior(Q)[:, 5] - sur.interior(Qn)[:, 5]).abs().max().item()
+    dY0 = (sur.mass_fractions(Qn, rn) - sur.mass_fractions(Q0, r0)).abs().max().item()
+    times = {}
+    for name in ("newton", "surrogate"):
+        ts = []
+        for _ in range(3):
+            Qf, rf, Uf = Q0.clone(), r0.clone(), torch.zeros_like(U)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            if name == "newton":
+                ctx.chem_newton(Uf, Qf, rf, sur.AIR_DT, 50, 1e-10, 1e-6)
+            else:
+                ctx.nn_chem(Uf, Qf, rf, sur.AIR_DT, 50, 1e-10, 1e-6)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        times[name] = min(ts)
+    frac = nfb / st.numel()
+    print(f"  Y linf {Ye:.2e} (step's own change {dY0:.2e})  T linf {Te:.3f} K  fallback "
+          f"{frac:.4f}  newton {times['newton']:.2f} ms  surrogate {times['surrogate']:.2f} ms",
+          flush=True)
+    assert dY0 > 1e-4  # the step does move the composition
+    assert Ye <= 1e-3
+    assert Te <= 1.0
+    assert frac <= 0.5
     ctx.close()
 
 

@@ -48,15 +48,16 @@ def test_gelu_closed_forms():
 
 
 @pytest.mark.parametrize("layer", [0, 1, 2, 3])
-def test_gradient_finite_differences(layer):
+@pytest.mark.parametrize("nin,nout", [(11, 9), (7, 5)])  # H2/O2, 5-species air
+def test_gradient_finite_differences(layer, nin, nout):
     rng = np.random.default_rng(1 + layer)
     p = _rand_params(layer)
-    X = rng.normal(0, 1, (16, mlp.NN_IN))
-    T = rng.normal(0, 1, (16, mlp.NN_OUT))
+    X = rng.normal(0, 1, (16, nin))
+    T = rng.normal(0, 1, (16, nout))
     _, g = mlp.loss_grad(p, X, T)
     ni, no = mlp.LAYERS[layer]
-    fi = 11 if ni == mlp.NN_INP else ni
-    fo = 9 if no == mlp.NN_OUTP else no
+    fi = nin if ni == mlp.NN_INP else ni
+    fo = nout if no == mlp.NN_OUTP else no
     idx = [mlp.W_OFF[layer] + o * ni + k for o, k in zip(rng.integers(0, fo, 12),
                                                           rng.integers(0, fi, 12))]
     idx += [mlp.B_OFF[layer] + o for o in rng.integers(0, fo, 6)]
@@ -69,11 +70,11 @@ def test_gradient_finite_differences(layer):
         assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
     # padded weights carry no gradient
     if ni == mlp.NN_INP:
-        pad = [mlp.W_OFF[0] + o * ni + k for o in range(no) for k in range(11, ni)]
+        pad = [mlp.W_OFF[0] + o * ni + k for o in range(no) for k in range(nin, ni)]
         assert np.all(g[pad] == 0)
     if no == mlp.NN_OUTP:
-        pad = list(range(mlp.W_OFF[3] + 9 * ni, mlp.W_OFF[3] + no * ni))
-        pad += list(range(mlp.B_OFF[3] + 9, mlp.B_OFF[3] + no))
+        pad = list(range(mlp.W_OFF[3] + nout * ni, mlp.W_OFF[3] + no * ni))
+        pad += list(range(mlp.B_OFF[3] + nout, mlp.B_OFF[3] + no))
         assert np.all(g[pad] == 0)
 
 
