@@ -244,7 +244,9 @@ typedef struct {
   double dt;             /* chemistry step the network predicts */
 } f3d_nn_config;
 /* Allocates the network (parameters zero) and stores the configuration;
- * the context must have Ns = 9.  May be called again to change the config.
+ * the context must have Ns = 9 or Ns = 5 (inputs Ns + 2, outputs Ns; the
+ * arrays above hold 11 and 9 entries, of which the first Ns + 2 and Ns are
+ * used).  May be called again to change the config.
  * With n_elem > 0 every prediction is projected onto the element-conserving
  * subspace after the element-balance gate (weighted by out_std^2, or per cell
  * in label_mode 3). */
