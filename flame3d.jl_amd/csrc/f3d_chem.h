@@ -198,13 +198,30 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
     const double Kc = ipow(RsT, P->sgm[n]) * exp(dg);
     const double kr = kf / Kc;
-    double q_f = kf;
-    double q_r = kr;
+    // q_f, q_r: the full mass-action products (same order as production_rate,
+    // so wdot is bit-identical to it); o_f, o_r: the products over the
+    // species that are not chain carriers, shared by the three derivatives
+    double q_f = kf, q_r = kr, o_f = kf, o_r = kr;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
       const int vf = P->vf[n][m], vr = P->vr[n][m];
-      if (vf != 0) q_f *= ipow(sc[m], vf);
-      if (vr != 0) q_r *= ipow(sc[m], vr);
+      const bool carrier = (m == ci[0]) || (m == ci[1]) || (m == ci[2]);
+      if (vf != 0) {
+        const double t = ipow(sc[m], vf);
+        q_f *= t;
+        if (!carrier) o_f *= t;
+      }
+      if (vr != 0) {
+        const double t = ipow(sc[m], vr);
+        q_r *= t;
+        if (!carrier) o_r *= t;
+      }
+    }
+    int vfc[3], vrc[3];  // carrier stoichiometry of this reaction
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      vfc[j] = ci[j] >= 0 ? P->vf[n][ci[j]] : 0;
+      vrc[j] = ci[j] >= 0 ? P->vr[n][ci[j]] : 0;
     }
     const int rt = P->rtype[n];
     double dlnq_dM = 0.0;
@@ -246,17 +263,18 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     for (int j = 0; j < 3; ++j) {
       const int cj = ci[j];
       if (cj < 0) continue;
-      const int vfj = P->vf[n][cj], vrj = P->vr[n][cj];
+      const int vfj = vfc[j], vrj = vrc[j];
       double d = (rt != 1) ? dq * dlnq_dM * P->ef[n][cj] : 0.0;
       if (vfj != 0 || vrj != 0) {
-        double pf = (vfj != 0) ? kf * (double)vfj * ipow(cc[j], vfj - 1) : 0.0;
-        double pr = (vrj != 0) ? kr * (double)vrj * ipow(cc[j], vrj - 1) : 0.0;
+        // d(k c_j^v_j prod_{m != j} c_m^v_m)/dc_j: the non-carrier product
+        // times the other two carriers' powers
+        double pf = (vfj != 0) ? o_f * (double)vfj * ipow(cc[j], vfj - 1) : 0.0;
+        double pr = (vrj != 0) ? o_r * (double)vrj * ipow(cc[j], vrj - 1) : 0.0;
 #pragma unroll
-        for (int m = 0; m < NS; ++m) {
-          if (m == cj) continue;
-          const int vf = P->vf[n][m], vr = P->vr[n][m];
-          if (vf != 0) pf *= ipow(sc[m], vf);
-          if (vr != 0) pr *= ipow(sc[m], vr);
+        for (int k = 0; k < 3; ++k) {
+          if (k == j) continue;
+          if (vfc[k] != 0) pf *= ipow(cc[k], vfc[k]);
+          if (vrc[k] != 0) pr *= ipow(cc[k], vrc[k]);
         }
         d += (pf - pr) * fac;
       }
