@@ -679,8 +679,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   if (lane == 0) tile_count[tile] = (int32_t)__popcll(bal);
 }
 
-// Exclusive scan of the per-tile rejected counts (one workgroup of 1024
-// threads, each over a contiguous chunk), in place; *total = their sum.
+// Exclusive scan of n counts in place (one workgroup of 1024 threads, each
+// over a contiguous run of entries); *total = their sum.
 __global__ void __launch_bounds__(1024) k_nn_scan(int32_t* __restrict__ cnt, long n,
                                                    int32_t* __restrict__ total) {
   __shared__ int32_t part[1024];
@@ -704,6 +704,56 @@ __global__ void __launch_bounds__(1024) k_nn_scan(int32_t* __restrict__ cnt, lon
     run += c;
   }
   if (t == 1023) *total = part[1023];
+}
+
+// The tile counts are scanned in chunks of kScanChunk tiles: k_scan_sums
+// writes each chunk's sum, k_nn_scan scans the chunk sums (one workgroup),
+// k_scan_apply turns each chunk into exclusive offsets from its chunk's
+// offset.  Three short launches in place of one workgroup walking all tiles.
+constexpr int kScanThreads = 256, kScanPer = 4, kScanChunk = kScanThreads * kScanPer;
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_sums(const int32_t* __restrict__ cnt, long n, int32_t* __restrict__ sums) {
+  __shared__ int32_t part[kScanThreads];
+  const int t = threadIdx.x;
+  const long b = (long)blockIdx.x * kScanChunk + (long)t * kScanPer;
+  int32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) v += (b + i < n) ? cnt[b + i] : 0;
+  part[t] = v;
+  __syncthreads();
+  for (int off = kScanThreads / 2; off > 0; off >>= 1) {
+    if (t < off) part[t] += part[t + off];
+    __syncthreads();
+  }
+  if (t == 0) sums[blockIdx.x] = part[0];
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_apply(int32_t* __restrict__ cnt, long n, const int32_t* __restrict__ offs) {
+  __shared__ int32_t part[kScanThreads];
+  const int t = threadIdx.x;
+  const long b = (long)blockIdx.x * kScanChunk + (long)t * kScanPer;
+  int32_t v[kScanPer], s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    v[i] = (b + i < n) ? cnt[b + i] : 0;
+    s += v[i];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int32_t x = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int32_t run = offs[blockIdx.x] + part[t] - s;  // exclusive prefix of this thread's tiles
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (b + i < n) cnt[b + i] = run;
+    run += v[i];
+  }
 }
 
 // Rejected cell ids of each 64-cell tile at the tile's offset, in cell order.
@@ -884,6 +934,7 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   }
   if (!c->nn_ready) {
     const long ncell = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+    const long ntile = (ncell + 63) / 64, nchunk = (ntile + kScanChunk - 1) / kScanChunk;
     bool okm = hipMalloc(&c->nn_master, sizeof(float) * NN_NPARAM) == hipSuccess &&
                hipMalloc(&c->nn_m1, sizeof(float) * NN_NPARAM) == hipSuccess &&
                hipMalloc(&c->nn_m2, sizeof(float) * NN_NPARAM) == hipSuccess &&
@@ -891,7 +942,7 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
                hipMalloc(&c->nn_Wtb, 2 * NN_NPARAM) == hipSuccess &&
                hipMalloc(&c->nn_cfg, sizeof(NNDev)) == hipSuccess &&
                hipMalloc(&c->nn_status, ncell) == hipSuccess &&
-               hipMalloc(&c->nn_fbcount, sizeof(int32_t) * ((ncell + 63) / 64)) == hipSuccess &&
+               hipMalloc(&c->nn_fbcount, sizeof(int32_t) * (ntile + nchunk)) == hipSuccess &&
                hipMalloc(&c->nn_fblist, sizeof(int32_t) * ncell) == hipSuccess &&
                hipMalloc(&c->nn_fbtotal, sizeof(int32_t)) == hipSuccess;
     if (!okm) {
@@ -1028,7 +1079,14 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
     hipLaunchKernelGGL((k_nn_chem<5>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
                        c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
                        c->g, dt, c->nn_fbcount, st);
-  hipLaunchKernelGGL(k_nn_scan, dim3(1), dim3(1024), 0, s, c->nn_fbcount, ntile, c->nn_fbtotal);
+  // tile counts -> tile offsets; the chunk sums sit after the tile counts
+  const long nchunk = (ntile + kScanChunk - 1) / kScanChunk;
+  int32_t* sums = c->nn_fbcount + ntile;
+  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nchunk), dim3(kScanThreads), 0, s,
+                     c->nn_fbcount, ntile, sums);
+  hipLaunchKernelGGL(k_nn_scan, dim3(1), dim3(1024), 0, s, sums, nchunk, c->nn_fbtotal);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nchunk), dim3(kScanThreads), 0, s,
+                     c->nn_fbcount, ntile, sums);
   hipLaunchKernelGGL(k_nn_compact, dim3((unsigned)ntile), dim3(64), 0, s, st, c->nn_fbcount,
                      ncell, c->nn_fblist);
   rc = hip_check(hipGetLastError(), "f3d_nn_chem");
