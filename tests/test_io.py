@@ -1,7 +1,9 @@
-"""Plot files (.vts, VTK XML structured grid, appended raw) with the fields
-of the reference's plot output (solver.jl:394-434): write, read back, and
-compare with the state they came from (Float32 rounding of the FP64 state,
-Int64 tag exact, column-major i-fastest layout, the time as field data)."""
+"""Output formats.  Plot files (.vts, VTK XML structured grid, appended
+raw) with the fields of the reference's plot output (solver.jl:394-434):
+write, read back, and compare with the state they came from (Float32
+rounding of the FP64 state, Int64 tag exact, column-major i-fastest layout,
+the time as field data).  Restart files with the reference's checkpoint
+datasets (solver.jl:437-465) round-trip bit for bit."""
 import numpy as np
 import torch
 
@@ -40,3 +42,25 @@ def test_vts_round_trip(tmp_path, pkg):
     # x index fastest in the payload: the first nx values of rho are the first row
     raw = f["rho"].reshape(-1)
     assert np.array_equal(raw[:N[0]], Q[0][NG, NG, NG:-NG].numpy().astype(np.float32))
+
+
+def test_checkpoint_round_trip(tmp_path, pkg):
+    """Restart files: the reference's dataset names and [Nx_tot, Ny_tot,
+    Nz_tot, NV, Nprocs] column-major shapes (solver.jl:437-465), FP64 bit
+    for bit back into the library's layout."""
+    import flame3d_amd.cases as cases
+    import flame3d_amd.checkpoint as chk
+    import flame3d_amd.mech as mm
+    m = mm.load("h2_lidryer")
+    N = (9, 7, 5)
+    grid = cases.stretched_y_grid(*N, 1.0, 2.0, 0.5)
+    Q, rhoi = cases.random_state(grid, m, seed=5)
+    path = str(tmp_path / "chk100.npz")
+    chk.save_chk(path, Q, rhoi, step=100, time=2e-5)
+    z = np.load(path, allow_pickle=False)
+    assert z["Q_h"].shape == (N[0] + 8, N[1] + 8, N[2] + 8, 7, 1)
+    assert z["ρi_h"].shape == (N[0] + 8, N[1] + 8, N[2] + 8, m.Ns, 1)
+    assert z["Q_h"][5, 3, 2, 4, 0] == Q[4, 2, 3, 5].item()  # [i, j, k, var] indexing
+    Q2, r2, step, t = chk.load_chk(path, device="cpu")
+    assert step == 100 and t == 2e-5
+    assert torch.equal(Q2, Q) and torch.equal(r2, rhoi)
