@@ -110,6 +110,8 @@ _PROTOS = {
     "f3d_nn_chem": [_vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_double, C.c_double, _vp,
                     C.POINTER(C.c_int64), _vp, _vp],
     "f3d_set_metrics": [_vp, _vp, _vp],
+    "f3d_mesh_ghosts": [_vp, _vp, _vp, _vp, _vp],
+    "f3d_mesh_metrics": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "f3d_rk3_stage": [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp],
     "f3d_workspace": [_vp, C.c_char_p],
     "f3d_time_step": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
@@ -441,6 +443,22 @@ class Context:
         ptrs = self._mptrs(metrics)
         _rc(self.lib.f3d_set_metrics(self.h, ptrs, self._pad(J, None, "J")), "f3d_set_metrics")
         self._metrics_keep = (list(metrics), J)
+
+    def mesh_ghosts(self, x, y, z):
+        """Fill the ghost nodes of padded coordinates in place (parse_mesh.jl:52-113)."""
+        _rc(self.lib.f3d_mesh_ghosts(self.h, self._pad(x, None, "x"), self._pad(y, None, "y"),
+                                     self._pad(z, None, "z"), self.stream()), "f3d_mesh_ghosts")
+
+    def mesh_metrics(self, x, y, z):
+        """(9 cofactor metrics, J) at every padded point from padded node
+        coordinates (parse_mesh.jl:132-221), as new device arrays."""
+        metrics = [self.zeros() for _ in range(9)]
+        J = self.zeros()
+        _rc(self.lib.f3d_mesh_metrics(self.h, self._pad(x, None, "x"), self._pad(y, None, "y"),
+                                      self._pad(z, None, "z"), self._mptrs(metrics),
+                                      self._pad(J, None, "J"), self.stream()),
+            "f3d_mesh_metrics")
+        return metrics, J
 
     def rk3_stage(self, KRK, U, Un, rhoi, rhon, Q, dt, tag=None):
         if self._metrics_keep is None:

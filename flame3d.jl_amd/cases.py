@@ -186,6 +186,46 @@ def stretched_y_grid(Nx, Ny, Nz, Lx, Ly, Lz, beta=2.5, dev="cpu") -> Grid:
     return Grid(Nx, Ny, Nz, I * hx, y, K * hz, m, J)
 
 
+def parse_mesh_coords(Nx=300, Ny=127, Nz=127, Lx=0.02, ymin=-0.0032, ymax=0.0032, ystar=0.0,
+                      zmin=-0.0032, zmax=0.0032, zstar=0.0, alpha=4e-4, n_uniform=None,
+                      dev="cpu"):
+    """Interior node coordinates of the reference's mesh generator
+    (parse_mesh.jl:24-50, its defaults): x uniform over the first
+    n_uniform = Nx - 20 nodes with spacing Lx/(Nx-1), then growing by
+    (0.5 + m/2) spacings; y and z clustered around ystar / zstar by
+    y = ystar + alpha sinh(c1 (1 - s) + c2 s), s = j/(Ny-1),
+    c1 = asinh((ymin - ystar)/alpha), c2 = asinh((ymax - ystar)/alpha).
+    Returns padded (Nz+8, Ny+8, Nx+8) arrays with zero ghosts (f3d_mesh_ghosts
+    fills them)."""
+    n_uniform = Nx - 20 if n_uniform is None else n_uniform
+    h = Lx / (Nx - 1)
+    xs = [i * h for i in range(n_uniform)]
+    for i in range(n_uniform + 1, Nx + 1):  # 1-based i, as the loop it follows
+        xs.append(xs[-1] + h * (0.5 + (i - n_uniform) / 2))
+    c1, c2 = math.asinh((ymin - ystar) / alpha), math.asinh((ymax - ystar) / alpha)
+    c3, c4 = math.asinh((zmin - zstar) / alpha), math.asinh((zmax - zstar) / alpha)
+    ys = [ystar + alpha * math.sinh(c1 * (1 - j / (Ny - 1)) + c2 * j / (Ny - 1)) for j in range(Ny)]
+    zs = [zstar + alpha * math.sinh(c3 * (1 - k / (Nz - 1)) + c4 * k / (Nz - 1)) for k in range(Nz)]
+    sh = (Nz + 2 * NG, Ny + 2 * NG, Nx + 2 * NG)
+    x = torch.zeros(sh, dtype=torch.float64)
+    y = torch.zeros(sh, dtype=torch.float64)
+    z = torch.zeros(sh, dtype=torch.float64)
+    it = slice(NG, NG + Nz), slice(NG, NG + Ny), slice(NG, NG + Nx)
+    x[it] = torch.tensor(xs, dtype=torch.float64)[None, None, :]
+    y[it] = torch.tensor(ys, dtype=torch.float64)[None, :, None]
+    z[it] = torch.tensor(zs, dtype=torch.float64)[:, None, None]
+    return x.to(dev), y.to(dev), z.to(dev)
+
+
+def grid_from_coords(ctx, x, y, z) -> Grid:
+    """Ghost nodes and metrics of padded node coordinates on the GPU
+    (f3d_mesh_ghosts, f3d_mesh_metrics); x, y, z are device arrays whose
+    ghosts are overwritten."""
+    ctx.mesh_ghosts(x, y, z)
+    m, J = ctx.mesh_metrics(x, y, z)
+    return Grid(ctx.Nx, ctx.Ny, ctx.Nz, x, y, z, m, J)
+
+
 # ---------------------------------------------------------------------------
 # initial states
 # ---------------------------------------------------------------------------

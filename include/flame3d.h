@@ -285,6 +285,22 @@ int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int
  * given these same pointers read instead (the values the kernels would
  * convert to anyway).  Call again after changing the arrays' contents. */
 int f3d_set_metrics(f3d_ctx* ctx, const double* const* metrics, const double* J);
+
+/* ---- mesh (parse_mesh.jl) ----------------------------------------------- */
+/* Ghost node coordinates of padded x, y, z (interior nodes given), in place:
+ * point reflection through the boundary node along x and y for interior k,
+ * the x-y corner columns from the two adjacent ghost rows, then the z ghost
+ * planes over the whole x-y extent (generateXYZ, parse_mesh.jl:52-113).
+ * Needs Nx, Ny, Nz >= 5. */
+int f3d_mesh_ghosts(f3d_ctx* ctx, double* x, double* y, double* z, hipStream_t s);
+/* Cofactor metrics (9 device arrays, order as f3d_set_metrics) and J at
+ * every padded point from padded node coordinates (computeMetrics,
+ * parse_mesh.jl:132-221): sixth-order central differences (CD6) of x, y, z
+ * along each index, second-order one-sided (CD2_L / CD2_R) in the three
+ * outermost layers; J = 1/det of the coordinate Jacobian; metric (a, b) =
+ * d(index a)/d(coordinate b) / J. */
+int f3d_mesh_metrics(f3d_ctx* ctx, const double* x, const double* y, const double* z,
+                     double* const* metrics, double* J, hipStream_t s);
 /* One Shu-Osher RK3 stage KRK = 1, 2, 3 exactly as solver.jl:298-326:
  * (KRK == 1: Un <- U, rhon <- rhoi), mixture, shockSensor, specAdvance,
  * flowAdvance, linComb (KRK 2, 3), c2Prim, fills.  Intermediates live in
