@@ -1,4 +1,4 @@
-// Learned chemistry surrogate: an 11 -> 64 -> 64 -> 64 -> 9 MLP with GELU
+// Learned chemistry surrogate: an (Ns+2) -> 64 -> 64 -> 64 -> Ns MLP with GELU
 // (tanh form), BF16 weights and activations on the gfx950 matrix cores
 // (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, trained on the GPU by the
 // kernels below (forward, backward, Adam) and evaluated in a fused chemistry
@@ -6,7 +6,8 @@
 // balance of the prediction) and hands rejected cells to the FP64 Newton
 // integrator.  The reference pattern is reactions.jl:2-72 (pre_input,
 // evalModel, post_predict); the network shape and data follow the 128^3
-// H2/O2 chemistry workload.
+// H2/O2 chemistry workload (Ns = 9); the same kernels, with inputs padded to
+// 32 and outputs to 32, serve 5-species air (Ns = 5).
 //
 // MFMA operand layout of v_mfma_f32_16x16x32_bf16 (verified on the device):
 // lane l holds A[l%16][8*(l/16) + i] and B[8*(l/16) + i][l%16] (i = 0..7) and
