@@ -265,6 +265,11 @@ static inline double dot5(double lgT, double lgT2, double lgT3, double lgT4, con
   return p[0] + lgT * p[1] + lgT2 * p[2] + lgT3 * p[3] + lgT4 * p[4];
 }
 
+// 1: mixture-averaged D_k with the reference's literal numerator 1 - X_k
+// (thermo.jl:359-368); 0 (default): sum_{l != k} X_l, the form the library uses
+static int g_diff_reference_form = 0;
+void or_set_diffusivity_reference_form(int32_t on) { g_diff_reference_form = on ? 1 : 0; }
+
 void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, double P,
                            const double* X, double* Diff, double* lambda_out, double* mu_out) {
   Grid g(d);
@@ -316,7 +321,8 @@ void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, doub
   }
   // thermo.jl:359-368 with 1 - X_k evaluated as sum_{l!=k} X_l (equal in
   // exact arithmetic since sum X = 1; the difference cancels catastrophically
-  // in nearly pure cells, where it is rounding noise)
+  // in nearly pure cells, where it is rounding noise).  g_diff_reference_form
+  // selects the reference's literal 1 - X_k, for measuring that difference.
   for (int n = 0; n < Ns; ++n) {
     double s = 0.0, sx = 0.0;
     for (int nn = 0; nn < Ns; ++nn) {
@@ -325,6 +331,7 @@ void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, doub
       sx += X[nn];
     }
     s *= P;
+    if (g_diff_reference_form) sx = 1 - X[n];
     Diff[n] = sx / (s + kEps);
   }
   *lambda_out = lambda;

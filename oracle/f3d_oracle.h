@@ -58,6 +58,7 @@ void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics
                         double* wdot, double* Arate /* [Ns,Ns] or NULL */);
 void or_gauss_solve(int32_t n, double* a /* [n,n] col-major, destroyed */,
                     double* b /* destroyed */, double* x);
+void or_set_diffusivity_reference_form(int32_t on);
 void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, double P,
                            const double* X, double* Diff, double* lambda, double* mu);
 

@@ -176,12 +176,19 @@ class Oracle:
         self.lib.or_gauss_solve(C.c_int32(len(b)), _f(A), _f(b), _f(x))
         return x
 
-    def mixture_properties(self, T, P, X):
+    def mixture_properties(self, T, P, X, reference_form=False):
+        """(lambda, mu, D[Ns]) at one state; reference_form takes the
+        mixture-averaged D_k with the reference's literal 1 - X_k numerator."""
         X = np.ascontiguousarray(X, dtype=np.float64)
         D = np.zeros(self.Ns)
         lam, mu = C.c_double(0), C.c_double(0)
-        self.lib.or_mixture_properties(C.byref(self.dims), C.byref(self.thermo), C.c_double(T),
-                                       C.c_double(P), _f(X), _f(D), C.byref(lam), C.byref(mu))
+        self.lib.or_set_diffusivity_reference_form(C.c_int32(1 if reference_form else 0))
+        try:
+            self.lib.or_mixture_properties(C.byref(self.dims), C.byref(self.thermo),
+                                           C.c_double(T), C.c_double(P), _f(X), _f(D),
+                                           C.byref(lam), C.byref(mu))
+        finally:
+            self.lib.or_set_diffusivity_reference_form(C.c_int32(0))
         return lam.value, mu.value, D
 
     def reactor0d_step(self, rhoi, E, dt, jac_eps, gt_tol, gt_maxiter, gt_tmin, gt_tmax):
