@@ -63,6 +63,324 @@ inline double ipow(double x, long n) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// The chemistry path (thermo, rates, Jacobian, pivoted solve, backward-Euler
+// Newton) as templates over the scalar type: instantiated with double for
+// the oracle itself and with CountedReal for the operation count of one
+// Newton solve (or_newton_opcount), so the count is taken from this very
+// operation sequence.
+// ---------------------------------------------------------------------------
+struct OpTally {
+  double add = 0, mul = 0, div = 0, trans = 0, sqrt = 0;
+};
+OpTally g_tally;
+
+// double that tallies its arithmetic: add/sub, mul, div, and transcendental
+// calls (exp, log, log10, pow) in g_tally; comparisons and fabs are free
+struct CountedReal {
+  double v = 0;
+  CountedReal() = default;
+  CountedReal(double x) : v(x) {}
+  explicit operator double() const { return v; }
+  CountedReal& operator+=(CountedReal b) { g_tally.add += 1; v += b.v; return *this; }
+  CountedReal& operator-=(CountedReal b) { g_tally.add += 1; v -= b.v; return *this; }
+  CountedReal& operator*=(CountedReal b) { g_tally.mul += 1; v *= b.v; return *this; }
+  CountedReal& operator/=(CountedReal b) { g_tally.div += 1; v /= b.v; return *this; }
+};
+using CR = CountedReal;
+inline CR operator+(CR a, CR b) { g_tally.add += 1; return CR(a.v + b.v); }
+inline CR operator-(CR a, CR b) { g_tally.add += 1; return CR(a.v - b.v); }
+inline CR operator*(CR a, CR b) { g_tally.mul += 1; return CR(a.v * b.v); }
+inline CR operator/(CR a, CR b) { g_tally.div += 1; return CR(a.v / b.v); }
+inline CR operator-(CR a) { return CR(-a.v); }
+inline bool operator<(CR a, CR b) { return a.v < b.v; }
+inline bool operator>(CR a, CR b) { return a.v > b.v; }
+inline bool operator<=(CR a, CR b) { return a.v <= b.v; }
+inline bool operator>=(CR a, CR b) { return a.v >= b.v; }
+inline bool operator==(CR a, CR b) { return a.v == b.v; }
+inline bool operator!=(CR a, CR b) { return a.v != b.v; }
+inline CR exp(CR a) { g_tally.trans += 1; return CR(std::exp(a.v)); }
+inline CR log(CR a) { g_tally.trans += 1; return CR(std::log(a.v)); }
+inline CR log10(CR a) { g_tally.trans += 1; return CR(std::log10(a.v)); }
+inline CR pow(CR a, CR b) { g_tally.trans += 1; return CR(std::pow(a.v, b.v)); }
+inline CR sqrt(CR a) { g_tally.sqrt += 1; return CR(std::sqrt(a.v)); }
+inline CR fabs(CR a) { return CR(std::fabs(a.v)); }
+inline double dval(double x) { return x; }
+inline double dval(CR x) { return x.v; }
+
+template <class D>
+D ipow_t(D x, long n) {
+  if (n == 0) return D(1.0);
+  if (n < 0) return D(1.0) / ipow_t(x, -n);
+  D r = x;
+  for (long m = 1; m < n; ++m) r = r * x;
+  return r;
+}
+
+// cp_specs, thermo.jl:82-104
+template <class D>
+void cp_specs_t(const Grid& g, const or_thermo* th, D T, D* cp) {
+  D T2 = T * T;
+  D T3 = T2 * T;
+  D T4 = T2 * T2;
+  for (int n = 0; n < g.Ns; ++n) {
+    const double* c = (T < th->coeffs_sep[n]) ? th->coeffs_lo + 7 * n : th->coeffs_hi + 7 * n;
+    cp[n] = c[0] + c[1] * T + c[2] * T2 + c[3] * T3 + c[4] * T4;
+  }
+}
+
+// ei_specs, thermo.jl:136-160 (e/R per species, K)
+template <class D>
+void ei_specs_t(const Grid& g, const or_thermo* th, D T, D* ei) {
+  D T2 = 0.5 * T * T;
+  D T3 = 2.0 / 3.0 * T2 * T;
+  D T4 = T2 * T2;
+  D T5 = 0.8 * T4 * T;
+  for (int n = 0; n < g.Ns; ++n) {
+    const double* lo = th->coeffs_lo + 7 * n;
+    const double* hc = th->coeffs_hi + 7 * n;
+    if (T < th->coeffs_sep[n]) {
+      ei[n] = (lo[0] - 1) * T + lo[1] * T2 + lo[2] * T3 + lo[3] * T4 + lo[4] * T5;
+    } else {
+      ei[n] = (hc[0] - 1) * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 +
+              (hc[5] - lo[5]);
+    }
+  }
+}
+
+// gibbs, thermo.jl:163-191 (g/RT)
+template <class D>
+void gibbs_t(const Grid& g, const or_thermo* th, D T, D lgT, D invT, D* gi) {
+  D mlogT = 1 - lgT;
+  D T1 = 0.5 * T;
+  D T2 = 1.0 / 6.0 * T * T;
+  D T3 = T1 * T2;
+  D T4 = 1.8 * T2 * T2;
+  for (int n = 0; n < g.Ns; ++n) {
+    const double* c = (T < th->coeffs_sep[n]) ? th->coeffs_lo + 7 * n : th->coeffs_hi + 7 * n;
+    gi[n] = c[0] * mlogT - c[1] * T1 - c[2] * T2 - c[3] * T3 - c[4] * T4 + c[5] * invT - c[6];
+  }
+}
+
+// CV, thermo.jl:194-203 (J/m^3/K)
+template <class D>
+D cv_t(const Grid& g, const or_thermo* th, D T, const D* rhoi) {
+  D cp[64];
+  cp_specs_t(g, th, T, cp);
+  D result = 0;
+  for (int n = 0; n < g.Ns; ++n) result += (cp[n] - 1) * rhoi[n] / th->mw[n];
+  return result * th->Ru;
+}
+
+// InternalEnergy, thermo.jl:219-228 (J/m^3)
+template <class D>
+D internal_energy_t(const Grid& g, const or_thermo* th, D T, const D* rhoi) {
+  D ei[64];
+  ei_specs_t(g, th, T, ei);
+  D result = 0;
+  for (int n = 0; n < g.Ns; ++n) result += rhoi[n] * ei[n] / th->mw[n];
+  return result * th->Ru;
+}
+
+// vproductionRate / vproductionRate_Jac, reactions.jl:311-460 (see or_production_rate)
+template <class D>
+void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D* sc, D T,
+             double jac_eps, int32_t full_jac, D* wdot, D* Arate) {
+  using std::exp;
+  using std::log;
+  using std::log10;
+  using std::pow;
+  const int Ns = g.Ns, Nr = g.Nr;
+  std::vector<D> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
+  for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
+  if (Arate)
+    for (int m = 0; m < Ns * Ns; ++m) Arate[m] = 0.0;
+  D lgT = log(T);
+  D invT = 1.0 / T;
+  for (int n = 0; n < Nr; ++n)
+    kf[n] = kin->Arr[3 * n] * exp(kin->Arr[3 * n + 1] * lgT - kin->Arr[3 * n + 2] * invT);
+  gibbs_t<D>(g, th, T, lgT, invT, gi.data());
+  D RsT = th->Ru / kin->atm * 1e6 * T;
+  for (int n = 0; n < Nr; ++n) {
+    D dg = 0;
+    for (int m = 0; m < Ns; ++m)
+      dg += gi[m] * (double)(kin->vf[m + Ns * n] - kin->vr[m + Ns * n]);
+    Kc[n] = ipow_t(RsT, kin->sgm[n]) * exp(dg);
+  }
+  for (int n = 0; n < Nr; ++n) {
+    D q_f = kf[n];
+    D q_r = q_f / Kc[n];
+    for (int m = 0; m < Ns; ++m) {
+      long vf = kin->vf[m + Ns * n], vr = kin->vr[m + Ns * n];
+      if (vf != 0) q_f *= ipow_t(sc[m], vf);
+      if (vr != 0) q_r *= ipow_t(sc[m], vr);
+    }
+    D mixture = 0.0;
+    D dlnq_dM = 0.0;  // d ln(q) / d[M], used by the full Jacobian only
+    kfac_den[n] = 0.0;
+    if (kin->reaction_type[n] == 2) {
+      for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
+      q_f *= mixture;
+      q_r *= mixture;
+      dlnq_dM = 1.0 / mixture;
+      kfac_den[n] = mixture;
+    } else if (kin->reaction_type[n] == 3) {
+      for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
+      const double* lp = kin->loP + 3 * n;
+      const double* tr = kin->Troe + 4 * n;
+      D redP = mixture / kf[n] * lp[0] * exp(lp[1] * lgT - lp[2] * invT);
+      D logPred = log10(redP);
+      double A = tr[0];
+      D logFcent =
+          log10((1 - A) * exp(-T / tr[1]) + A * exp(-T / tr[2]) + exp(-tr[3] * invT));
+      D troe_c = -0.4 - 0.67 * logFcent;
+      D troe_n = 0.75 - 1.27 * logFcent;
+      D troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
+      D F_troe = pow(D(10.0), logFcent / (1.0 + troe * troe));
+      D k_1 = (redP / (1.0 + redP)) * F_troe;
+      q_f *= k_1;
+      q_r *= k_1;
+      dlnq_dM = 1.0 / ((1.0 + redP) * mixture);
+      kfac_den[n] = k_1;
+    }
+    if (Arate && full_jac && dlnq_dM != 0.0) {
+      // third-body / falloff dependence through [M] = sum ef_m c_m
+      // (Lindemann part of the falloff; F_troe's dependence on Pr is neglected)
+      const D dq = (q_f - q_r) * dlnq_dM;
+      for (int m = 0; m < Ns; ++m) {
+        const double e = kin->ef[m + Ns * n];
+        if (e == 0.0) continue;
+        for (int l = 0; l < Ns; ++l)
+          Arate[l + Ns * m] += dq * e * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
+      }
+    }
+    if (Arate && full_jac) {
+      // exact concentration derivatives of the mass-action products:
+      // dq/dc_m = q0 * nu_m c_m^(nu_m-1) prod_{j != m} c_j^nu_j, q0 = q / prod
+      const D kfac = q_f, krac = q_r;  // after the [M] / falloff factors
+      for (int m = 0; m < Ns; ++m) {
+        long vfm = kin->vf[m + Ns * n], vrm = kin->vr[m + Ns * n];
+        if (vfm == 0 && vrm == 0) continue;
+        D dqf = 0.0, dqr = 0.0;
+        if (vfm != 0) {
+          D pr = kf[n] * (double)vfm * ipow_t(sc[m], vfm - 1);
+          for (int j = 0; j < Ns; ++j)
+            if (j != m && kin->vf[j + Ns * n] != 0) pr *= ipow_t(sc[j], kin->vf[j + Ns * n]);
+          dqf = pr;
+        }
+        if (vrm != 0) {
+          D pr = kf[n] / Kc[n] * (double)vrm * ipow_t(sc[m], vrm - 1);
+          for (int j = 0; j < Ns; ++j)
+            if (j != m && kin->vr[j + Ns * n] != 0) pr *= ipow_t(sc[j], kin->vr[j + Ns * n]);
+          dqr = pr;
+        }
+        // the third-body / falloff factor multiplies both directions
+        const D fac = (kfac_den[n] != 0.0) ? kfac_den[n] : D(1.0);
+        const D dd = (dqf - dqr) * fac;
+        for (int l = 0; l < Ns; ++l)
+          Arate[l + Ns * m] += dd * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
+        (void)kfac;
+        (void)krac;
+      }
+    }
+    for (int m = 0; m < Ns; ++m) {
+      wdot[m] += (q_f - q_r) * (double)(kin->vr[m + Ns * n] - kin->vf[m + Ns * n]);
+      if (Arate && !full_jac) {
+        D invsc = 1 / (sc[m] + jac_eps);
+        D Awf = (double)kin->vf[m + Ns * n] * q_f * invsc;
+        D Awr = (double)kin->vr[m + Ns * n] * q_r * invsc;
+        for (int l = 0; l < Ns; ++l)
+          Arate[l + Ns * m] += (Awf - Awr) * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
+      }
+    }
+  }
+}
+
+// solve, reactions.jl:463-503 (see or_gauss_solve)
+template <class D>
+void gauss_t(int32_t n, D* a, D* b, D* x) {
+  using std::fabs;
+  auto A = [&](int i, int j) -> D& { return a[i + n * j]; };
+  for (int k = 0; k < n - 1; ++k) {
+    D fmax = fabs(A(k, k));
+    int kk = k;
+    for (int i = k + 1; i < n; ++i) {
+      D absa = fabs(A(i, k));
+      if (absa > fmax) {
+        fmax = absa;
+        kk = i;
+      }
+    }
+    for (int j = k; j < n; ++j) std::swap(A(k, j), A(kk, j));
+    std::swap(b[k], b[kk]);
+    for (int i = k + 1; i < n; ++i) {
+      D f = -A(i, k) / A(k, k);
+      for (int j = 0; j < n; ++j) A(i, j) += f * A(k, j);
+      b[i] += f * b[k];
+    }
+  }
+  x[n - 1] = b[n - 1] / A(n - 1, n - 1);
+  for (int i = n - 2; i >= 0; --i) {
+    x[i] = b[i];
+    for (int j = i + 1; j < n; ++j) x[i] -= A(i, j) * x[j];
+    x[i] /= A(i, i);
+  }
+}
+
+// backward-Euler Newton solve of or_chem_newton (see be_solve)
+template <class D>
+bool be_solve_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, D* r, D& T,
+                double h, double E0, const double* hf, double rho_tot, double T_lo,
+                double T_hi, int maxiter, double rtol, double ttol, int& it) {
+  using std::fabs;
+  const int Ns = g.Ns;
+  const int N1 = Ns + 1;
+  D rs[64], s[64], wdot[64], wdot2[64], eis[64], rhs[64], dx[64];
+  std::vector<D> Arate(Ns * Ns), A(N1 * N1);
+  for (int n = 0; n < Ns; ++n) rs[n] = r[n];
+  D dT_prev = 0.0;
+  for (int k = 0; k < maxiter; ++k) {
+    for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
+    prate_t<D>(g, th, kin, s, T, kEps, 1, wdot, Arate.data());
+    D hT = 1e-6 * T;
+    prate_t<D>(g, th, kin, s, T + hT, kEps, 1, wdot2, static_cast<D*>(nullptr));
+    for (int n = 0; n < Ns; ++n) {
+      D src = wdot[n] * th->mw[n] * 1e6 * h;
+      rhs[n] = -(r[n] - rs[n] - src);
+      for (int l = 0; l < Ns; ++l)
+        A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * h;
+      A[n + N1 * Ns] = -((wdot2[n] - wdot[n]) / hT) * th->mw[n] * 1e6 * h;
+    }
+    D E = internal_energy_t<D>(g, th, T, r);
+    for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
+    rhs[Ns] = -(E - E0);
+    ei_specs_t<D>(g, th, T, eis);
+    for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * th->Ru / th->mw[l] + hf[l];
+    A[Ns + N1 * Ns] = cv_t<D>(g, th, T, r);
+    gauss_t<D>(N1, A.data(), rhs, dx);
+    D rho = 0;
+    for (int n = 0; n < Ns; ++n) rho += r[n];
+    double mx = 0;
+    bool bad = false;
+    for (int n = 0; n < Ns; ++n) {
+      r[n] += dx[n];
+      mx = std::max(mx, dval(fabs(dx[n])));
+      bad = bad || !(r[n] > -rho_tot);
+    }
+    T += dx[Ns];
+    ++it;
+    // NASA-7 polynomials jump at T_mid: when E0 falls in such a jump the T
+    // update flips sign without contracting; with the species converged the
+    // step is accepted (the final T comes from GetT of the energy)
+    const bool flip = k > 0 && dval(dx[Ns]) * dval(dT_prev) < 0 &&
+                      std::fabs(dval(dx[Ns])) >= 0.5 * std::fabs(dval(dT_prev));
+    dT_prev = dx[Ns];
+    if (bad || !(T > T_lo && T < T_hi)) return false;
+    if (mx <= rtol * dval(rho) && (std::fabs(dval(dx[Ns])) <= ttol || flip)) return true;
+  }
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -81,13 +399,7 @@ int32_t or_num_threads(void) {
 
 // cp_specs, thermo.jl:82-104
 static void cp_specs(const Grid& g, const or_thermo* th, double T, double* cp) {
-  double T2 = T * T;
-  double T3 = T2 * T;
-  double T4 = T2 * T2;
-  for (int n = 0; n < g.Ns; ++n) {
-    const double* c = (T < th->coeffs_sep[n]) ? th->coeffs_lo + 7 * n : th->coeffs_hi + 7 * n;
-    cp[n] = c[0] + c[1] * T + c[2] * T2 + c[3] * T3 + c[4] * T4;
-  }
+  cp_specs_t<double>(g, th, T, cp);
 }
 
 // h_specs, thermo.jl:107-133 (J/kg, formation enthalpy excluded)
@@ -111,34 +423,13 @@ void or_h_specs(const or_dims* d, const or_thermo* th, double T, double* hi) {
 
 // ei_specs, thermo.jl:136-160 (e/R per species, K)
 static void ei_specs(const Grid& g, const or_thermo* th, double T, double* ei) {
-  double T2 = 0.5 * T * T;
-  double T3 = 2.0 / 3.0 * T2 * T;
-  double T4 = T2 * T2;
-  double T5 = 0.8 * T4 * T;
-  for (int n = 0; n < g.Ns; ++n) {
-    const double* lo = th->coeffs_lo + 7 * n;
-    const double* hc = th->coeffs_hi + 7 * n;
-    if (T < th->coeffs_sep[n]) {
-      ei[n] = (lo[0] - 1) * T + lo[1] * T2 + lo[2] * T3 + lo[3] * T4 + lo[4] * T5;
-    } else {
-      ei[n] = (hc[0] - 1) * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 +
-              (hc[5] - lo[5]);
-    }
-  }
+  ei_specs_t<double>(g, th, T, ei);
 }
 
 // gibbs, thermo.jl:163-191 (g/RT)
 static void gibbs_impl(const Grid& g, const or_thermo* th, double T, double lgT, double invT,
                        double* gi) {
-  double mlogT = 1 - lgT;
-  double T1 = 0.5 * T;
-  double T2 = 1.0 / 6.0 * T * T;
-  double T3 = T1 * T2;
-  double T4 = 1.8 * T2 * T2;
-  for (int n = 0; n < g.Ns; ++n) {
-    const double* c = (T < th->coeffs_sep[n]) ? th->coeffs_lo + 7 * n : th->coeffs_hi + 7 * n;
-    gi[n] = c[0] * mlogT - c[1] * T1 - c[2] * T2 - c[3] * T3 - c[4] * T4 + c[5] * invT - c[6];
-  }
+  gibbs_t<double>(g, th, T, lgT, invT, gi);
 }
 
 void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g) {
@@ -148,22 +439,12 @@ void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g) {
 
 // CV, thermo.jl:194-203 (J/m^3/K)
 double or_cv(const or_dims* d, const or_thermo* th, double T, const double* rhoi) {
-  Grid g(d);
-  double cp[64];
-  cp_specs(g, th, T, cp);
-  double result = 0;
-  for (int n = 0; n < g.Ns; ++n) result += (cp[n] - 1) * rhoi[n] / th->mw[n];
-  return result * th->Ru;
+  return cv_t<double>(Grid(d), th, T, rhoi);
 }
 
 // InternalEnergy, thermo.jl:219-228 (J/m^3)
 double or_internal_energy(const or_dims* d, const or_thermo* th, double T, const double* rhoi) {
-  Grid g(d);
-  double ei[64];
-  ei_specs(g, th, T, ei);
-  double result = 0;
-  for (int n = 0; n < g.Ns; ++n) result += rhoi[n] * ei[n] / th->mw[n];
-  return result * th->Ru;
+  return internal_energy_t<double>(Grid(d), th, T, rhoi);
 }
 
 // Pmixture, thermo.jl:20-26
@@ -348,141 +629,13 @@ void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, doub
 void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
                         const double* sc, double T, double jac_eps, int32_t full_jac,
                         double* wdot, double* Arate) {
-  Grid g(d);
-  const int Ns = g.Ns, Nr = g.Nr;
-  std::vector<double> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
-  for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
-  if (Arate)
-    for (int m = 0; m < Ns * Ns; ++m) Arate[m] = 0.0;
-  double lgT = std::log(T);
-  double invT = 1.0 / T;
-  for (int n = 0; n < Nr; ++n)
-    kf[n] = kin->Arr[3 * n] * std::exp(kin->Arr[3 * n + 1] * lgT - kin->Arr[3 * n + 2] * invT);
-  gibbs_impl(g, th, T, lgT, invT, gi.data());
-  double RsT = th->Ru / kin->atm * 1e6 * T;
-  for (int n = 0; n < Nr; ++n) {
-    double dg = 0;
-    for (int m = 0; m < Ns; ++m)
-      dg += gi[m] * (double)(kin->vf[m + Ns * n] - kin->vr[m + Ns * n]);
-    Kc[n] = ipow(RsT, kin->sgm[n]) * std::exp(dg);
-  }
-  for (int n = 0; n < Nr; ++n) {
-    double q_f = kf[n];
-    double q_r = q_f / Kc[n];
-    for (int m = 0; m < Ns; ++m) {
-      long vf = kin->vf[m + Ns * n], vr = kin->vr[m + Ns * n];
-      if (vf != 0) q_f *= ipow(sc[m], vf);
-      if (vr != 0) q_r *= ipow(sc[m], vr);
-    }
-    double mixture = 0.0;
-    double dlnq_dM = 0.0;  // d ln(q) / d[M], used by the full Jacobian only
-    kfac_den[n] = 0.0;
-    if (kin->reaction_type[n] == 2) {
-      for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
-      q_f *= mixture;
-      q_r *= mixture;
-      dlnq_dM = 1.0 / mixture;
-      kfac_den[n] = mixture;
-    } else if (kin->reaction_type[n] == 3) {
-      for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
-      const double* lp = kin->loP + 3 * n;
-      const double* tr = kin->Troe + 4 * n;
-      double redP = mixture / kf[n] * lp[0] * std::exp(lp[1] * lgT - lp[2] * invT);
-      double logPred = std::log10(redP);
-      double A = tr[0];
-      double logFcent =
-          std::log10((1 - A) * std::exp(-T / tr[1]) + A * std::exp(-T / tr[2]) + std::exp(-tr[3] * invT));
-      double troe_c = -0.4 - 0.67 * logFcent;
-      double troe_n = 0.75 - 1.27 * logFcent;
-      double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
-      double F_troe = std::pow(10.0, logFcent / (1.0 + troe * troe));
-      double k_1 = (redP / (1.0 + redP)) * F_troe;
-      q_f *= k_1;
-      q_r *= k_1;
-      dlnq_dM = 1.0 / ((1.0 + redP) * mixture);
-      kfac_den[n] = k_1;
-    }
-    if (Arate && full_jac && dlnq_dM != 0.0) {
-      // third-body / falloff dependence through [M] = sum ef_m c_m
-      // (Lindemann part of the falloff; F_troe's dependence on Pr is neglected)
-      const double dq = (q_f - q_r) * dlnq_dM;
-      for (int m = 0; m < Ns; ++m) {
-        const double e = kin->ef[m + Ns * n];
-        if (e == 0.0) continue;
-        for (int l = 0; l < Ns; ++l)
-          Arate[l + Ns * m] += dq * e * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
-      }
-    }
-    if (Arate && full_jac) {
-      // exact concentration derivatives of the mass-action products:
-      // dq/dc_m = q0 * nu_m c_m^(nu_m-1) prod_{j != m} c_j^nu_j, q0 = q / prod
-      const double kfac = q_f, krac = q_r;  // after the [M] / falloff factors
-      for (int m = 0; m < Ns; ++m) {
-        long vfm = kin->vf[m + Ns * n], vrm = kin->vr[m + Ns * n];
-        if (vfm == 0 && vrm == 0) continue;
-        double dqf = 0.0, dqr = 0.0;
-        if (vfm != 0) {
-          double pr = kf[n] * (double)vfm * ipow(sc[m], vfm - 1);
-          for (int j = 0; j < Ns; ++j)
-            if (j != m && kin->vf[j + Ns * n] != 0) pr *= ipow(sc[j], kin->vf[j + Ns * n]);
-          dqf = pr;
-        }
-        if (vrm != 0) {
-          double pr = kf[n] / Kc[n] * (double)vrm * ipow(sc[m], vrm - 1);
-          for (int j = 0; j < Ns; ++j)
-            if (j != m && kin->vr[j + Ns * n] != 0) pr *= ipow(sc[j], kin->vr[j + Ns * n]);
-          dqr = pr;
-        }
-        // the third-body / falloff factor multiplies both directions
-        const double fac = (kfac_den[n] != 0.0) ? kfac_den[n] : 1.0;
-        const double dd = (dqf - dqr) * fac;
-        for (int l = 0; l < Ns; ++l)
-          Arate[l + Ns * m] += dd * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
-        (void)kfac;
-        (void)krac;
-      }
-    }
-    for (int m = 0; m < Ns; ++m) {
-      wdot[m] += (q_f - q_r) * (double)(kin->vr[m + Ns * n] - kin->vf[m + Ns * n]);
-      if (Arate && !full_jac) {
-        double invsc = 1 / (sc[m] + jac_eps);
-        double Awf = (double)kin->vf[m + Ns * n] * q_f * invsc;
-        double Awr = (double)kin->vr[m + Ns * n] * q_r * invsc;
-        for (int l = 0; l < Ns; ++l)
-          Arate[l + Ns * m] += (Awf - Awr) * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
-      }
-    }
-  }
+  prate_t<double>(Grid(d), th, kin, sc, T, jac_eps, full_jac, wdot, Arate);
 }
 
 // solve, reactions.jl:463-503: Gaussian elimination with partial pivoting,
 // a[i + n*j] column-major.
 void or_gauss_solve(int32_t n, double* a, double* b, double* x) {
-  auto A = [&](int i, int j) -> double& { return a[i + n * j]; };
-  for (int k = 0; k < n - 1; ++k) {
-    double fmax = std::fabs(A(k, k));
-    int kk = k;
-    for (int i = k + 1; i < n; ++i) {
-      double absa = std::fabs(A(i, k));
-      if (absa > fmax) {
-        fmax = absa;
-        kk = i;
-      }
-    }
-    for (int j = k; j < n; ++j) std::swap(A(k, j), A(kk, j));
-    std::swap(b[k], b[kk]);
-    for (int i = k + 1; i < n; ++i) {
-      double f = -A(i, k) / A(k, k);
-      for (int j = 0; j < n; ++j) A(i, j) += f * A(k, j);
-      b[i] += f * b[k];
-    }
-  }
-  x[n - 1] = b[n - 1] / A(n - 1, n - 1);
-  for (int i = n - 2; i >= 0; --i) {
-    x[i] = b[i];
-    for (int j = i + 1; j < n; ++j) x[i] -= A(i, j) * x[j];
-    x[i] /= A(i, i);
-  }
+  gauss_t<double>(n, a, b, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1269,51 +1422,9 @@ static bool be_solve(const Grid& g, const or_dims* d, const or_thermo* th,
                      const or_kinetics* kin, double* r, double& T, double h, double E0,
                      const double* hf, double rho_tot, double T_lo, double T_hi, int maxiter,
                      double rtol, double ttol, int& it) {
-  const int Ns = g.Ns;
-  const int N1 = Ns + 1;
-  double rs[64], s[64], wdot[64], wdot2[64], eis[64], rhs[64], dx[64];
-  std::vector<double> Arate(Ns * Ns), A(N1 * N1);
-  for (int n = 0; n < Ns; ++n) rs[n] = r[n];
-  double dT_prev = 0.0;
-  for (int k = 0; k < maxiter; ++k) {
-    for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
-    or_production_rate(d, th, kin, s, T, kEps, 1, wdot, Arate.data());
-    double hT = 1e-6 * T;
-    or_production_rate(d, th, kin, s, T + hT, kEps, 1, wdot2, nullptr);
-    for (int n = 0; n < Ns; ++n) {
-      double src = wdot[n] * th->mw[n] * 1e6 * h;
-      rhs[n] = -(r[n] - rs[n] - src);
-      for (int l = 0; l < Ns; ++l)
-        A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * h;
-      A[n + N1 * Ns] = -((wdot2[n] - wdot[n]) / hT) * th->mw[n] * 1e6 * h;
-    }
-    double E = or_internal_energy(d, th, T, r);
-    for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
-    rhs[Ns] = -(E - E0);
-    ei_specs(g, th, T, eis);
-    for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * th->Ru / th->mw[l] + hf[l];
-    A[Ns + N1 * Ns] = or_cv(d, th, T, r);
-    or_gauss_solve(N1, A.data(), rhs, dx);
-    double rho = 0;
-    for (int n = 0; n < Ns; ++n) rho += r[n];
-    double mx = 0;
-    bool bad = false;
-    for (int n = 0; n < Ns; ++n) {
-      r[n] += dx[n];
-      mx = std::max(mx, std::fabs(dx[n]));
-      bad = bad || !(r[n] > -rho_tot);
-    }
-    T += dx[Ns];
-    ++it;
-    // NASA-7 polynomials jump at T_mid: when E0 falls in such a jump the T
-    // update flips sign without contracting; with the species converged the
-    // step is accepted (the final T comes from GetT of the energy)
-    const bool flip = k > 0 && dx[Ns] * dT_prev < 0 && std::fabs(dx[Ns]) >= 0.5 * std::fabs(dT_prev);
-    dT_prev = dx[Ns];
-    if (bad || !(T > T_lo && T < T_hi)) return false;
-    if (mx <= rtol * rho && (std::fabs(dx[Ns]) <= ttol || flip)) return true;
-  }
-  return false;
+  (void)d;
+  return be_solve_t<double>(g, th, kin, r, T, h, E0, hf, rho_tot, T_lo, T_hi, maxiter, rtol, ttol,
+                            it);
 }
 
 void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double dt,
@@ -1428,6 +1539,81 @@ void or_rk3_stage(const or_dims* d, int32_t KRK, double* U, double* Un, double* 
   }
   or_c2prim(d, U, Q, rhoi, th, tag);
   or_fill(d, fill_kinds, Q, U, rhoi, inflow_Q, inflow_rhoi);
+}
+
+// ---------------------------------------------------------------------------
+// Operation counts of the chemistry path, from the same templates run on
+// CountedReal: out = [add/sub, mul, div, transcendental (exp, log, log10,
+// pow), sqrt, iterations].
+// ---------------------------------------------------------------------------
+static void tally_out(double* out, double iters) {
+  out[0] = g_tally.add;
+  out[1] = g_tally.mul;
+  out[2] = g_tally.div;
+  out[3] = g_tally.trans;
+  out[4] = g_tally.sqrt;
+  out[5] = iters;
+}
+
+// One backward-Euler solve over h of one reactor (rho_k, T), as be_solve
+// inside or_chem_newton (a single attempt, no substep restart).
+void or_newton_opcount(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                       const double* rhoi, double T0, double h, int32_t maxiter, double rtol,
+                       double ttol, double* out) {
+  Grid g(d);
+  const int Ns = g.Ns;
+  double hf[64], rho_tot = 0;
+  CR r[64];
+  for (int n = 0; n < Ns; ++n) {
+    hf[n] = th->coeffs_lo[5 + 7 * n] * th->Ru / th->mw[n];
+    rho_tot += rhoi[n];
+    r[n] = rhoi[n];
+  }
+  double E0 = or_internal_energy(d, th, T0, rhoi);
+  for (int n = 0; n < Ns; ++n) E0 += hf[n] * rhoi[n];
+  CR T = T0;
+  int it = 0;
+  g_tally = OpTally();
+  be_solve_t<CR>(g, th, kin, r, T, h, E0, hf, rho_tot, 0.5 * th->min_temp, 1.5 * th->max_temp,
+                 maxiter, rtol, ttol, it);
+  tally_out(out, it);
+}
+
+// One production-rate evaluation (reactions.jl:311-460): full_jac = 0 rates
+// only, 1 rates and the exact mass-action Jacobian.
+void or_rates_opcount(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                      const double* rhoi, double T, int32_t full_jac, double* out) {
+  Grid g(d);
+  const int Ns = g.Ns;
+  CR s[64], wdot[64];
+  std::vector<CR> A(Ns * Ns);
+  for (int n = 0; n < Ns; ++n) s[n] = rhoi[n] / th->mw[n] * 1e-6;
+  g_tally = OpTally();
+  prate_t<CR>(g, th, kin, s, CR(T), kEps, 1, wdot, full_jac ? A.data() : nullptr);
+  tally_out(out, 1);
+}
+
+// One e -> T Newton update (GetT, thermo.jl:231-282): InternalEnergy and CV
+// at T, the update (ein - e)/cv.
+void or_getT_update_opcount(const or_dims* d, const or_thermo* th, const double* rhoi, double T,
+                            double ein, double* out) {
+  Grid g(d);
+  CR r[64];
+  for (int n = 0; n < g.Ns; ++n) r[n] = rhoi[n];
+  g_tally = OpTally();
+  CR e1 = internal_energy_t<CR>(g, th, CR(T), r);
+  CR cv = cv_t<CR>(g, th, CR(T), r);
+  CR dT = (CR(ein) - e1) / cv;
+  (void)dT;
+  tally_out(out, 1);
+}
+
+// One pivoted solve of an n x n system (reactions.jl:463-503).
+void or_gauss_opcount(int32_t n, const double* a, const double* b, double* out) {
+  std::vector<CR> A(a, a + (size_t)n * n), B(b, b + n), X(n);
+  g_tally = OpTally();
+  gauss_t<CR>(n, A.data(), B.data(), X.data());
+  tally_out(out, 1);
 }
 
 // NN/H2/chemical_test.cpp:950-1007: T from GetT(E), rates and Jacobian,

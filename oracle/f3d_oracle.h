@@ -59,6 +59,16 @@ void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics
 void or_gauss_solve(int32_t n, double* a /* [n,n] col-major, destroyed */,
                     double* b /* destroyed */, double* x);
 void or_set_diffusivity_reference_form(int32_t on);
+/* operation counts of the chemistry path: out[6] = add/sub, mul, div,
+   transcendental, sqrt, iterations */
+void or_newton_opcount(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                       const double* rhoi, double T0, double h, int32_t maxiter, double rtol,
+                       double ttol, double* out);
+void or_rates_opcount(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                      const double* rhoi, double T, int32_t full_jac, double* out);
+void or_getT_update_opcount(const or_dims* d, const or_thermo* th, const double* rhoi, double T,
+                            double ein, double* out);
+void or_gauss_opcount(int32_t n, const double* a, const double* b, double* out);
 void or_mixture_properties(const or_dims* d, const or_thermo* th, double T, double P,
                            const double* X, double* Diff, double* lambda, double* mu);
 

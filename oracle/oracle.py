@@ -191,6 +191,42 @@ class Oracle:
             self.lib.or_set_diffusivity_reference_form(C.c_int32(0))
         return lam.value, mu.value, D
 
+    # ---- operation counts (the chemistry templates run on a counting type)
+    _OPS = ("add", "mul", "div", "trans", "sqrt", "iterations")
+
+    def _ops(self, out):
+        return dict(zip(self._OPS, (float(x) for x in out)))
+
+    def newton_opcount(self, rhoi, T, h, maxiter=50, rtol=1e-10, ttol=1e-6):
+        """Operations of one backward-Euler Newton solve of one reactor."""
+        out = np.zeros(6)
+        r = np.ascontiguousarray(rhoi, dtype=np.float64)
+        self.lib.or_newton_opcount(C.byref(self.dims), C.byref(self.thermo), C.byref(self.kin),
+                                   _f(r), C.c_double(T), C.c_double(h), C.c_int32(maxiter),
+                                   C.c_double(rtol), C.c_double(ttol), _f(out))
+        return self._ops(out)
+
+    def rates_opcount(self, rhoi, T, full_jac=False):
+        out = np.zeros(6)
+        r = np.ascontiguousarray(rhoi, dtype=np.float64)
+        self.lib.or_rates_opcount(C.byref(self.dims), C.byref(self.thermo), C.byref(self.kin),
+                                  _f(r), C.c_double(T), C.c_int32(int(full_jac)), _f(out))
+        return self._ops(out)
+
+    def getT_update_opcount(self, rhoi, T, ein):
+        out = np.zeros(6)
+        r = np.ascontiguousarray(rhoi, dtype=np.float64)
+        self.lib.or_getT_update_opcount(C.byref(self.dims), C.byref(self.thermo), _f(r),
+                                        C.c_double(T), C.c_double(ein), _f(out))
+        return self._ops(out)
+
+    def gauss_opcount(self, A, b):
+        out = np.zeros(6)
+        A = np.asfortranarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        self.lib.or_gauss_opcount(C.c_int32(len(b)), _f(A), _f(b), _f(out))
+        return self._ops(out)
+
     def reactor0d_step(self, rhoi, E, dt, jac_eps, gt_tol, gt_maxiter, gt_tmin, gt_tmax):
         Eb = C.c_double(E)
         T = C.c_double(0)
