@@ -328,6 +328,87 @@ def state_errors(ra, rb, mech):
             "per_species_Y_linf": {sp: float(dY[k].max()) for k, sp in enumerate(mech.species)}}
 
 
+ACC_BAR = {"T_K": 1.0, "Y": 1e-3}  # the accuracy bar of the FP32 + surrogate path
+
+
+def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6)):
+    """Untimed runs of `steps` time steps from the same initial state, each
+    compared after every step with FP64 plain (FP64 stage + FP64 Newton
+    chemistry) by the interior's max |dT| and max_k |dY_k|:
+      fp32_newton   FP32 stage + Newton chemistry   (the float stage's share)
+      fp64_nn       FP64 stage + surrogate          (the surrogate's share)
+      fp32_nn       FP32 stage + surrogate          (the benchmarked path)
+      fp64_perturbed  FP64 plain from velocities perturbed by 1e-7 relative
+                    (float rounding of the initial state): the flow's own
+                    sensitivity to a perturbation of that size.
+    Returns per-step curves, the values at `marks` and at the last step, and
+    the first step at which each path leaves the bar."""
+    N = case.grid.Nx, case.grid.Ny, case.grid.Nz
+    sl = (slice(4, -4),) * 3
+    keep_all = steps <= 40
+
+    def run(prec, nn, sink, perturb=0.0):
+        ctx = abi.Context(*N, mech, prec)
+        ctx.set_metrics(case.grid.metrics, case.grid.J)
+        ctx.set_boundary(case.bc)
+        Q, rhoi = case.Q.clone(), case.rhoi.clone()
+        if perturb:
+            g = torch.Generator(device="cuda").manual_seed(7)
+            for n in (1, 2, 3):
+                Q[n] *= 1 + perturb * torch.randn(Q[n].shape, generator=g, device="cuda",
+                                                  dtype=torch.float64)
+        U = ctx.zeros(5)
+        ctx.prim2c(U, Q)
+        Un, rn = ctx.zeros(5), ctx.zeros(mech.Ns)
+        if nn:
+            sur.attach(ctx)
+        chem = abi.CHEM_SURROGATE if nn else abi.CHEM_NEWTON
+        for s in range(1, steps + 1):
+            ctx.time_step(U, Un, rhoi, rn, Q, case.dt, chem, 1, 50, 1e-10, 1e-6)
+            sink(s, Q, rhoi)
+        torch.cuda.synchronize()
+        ctx.close()
+
+    def TY(Q, rhoi):
+        return Q[5][sl].clone(), (rhoi[(slice(None),) + sl] / Q[0][sl].unsqueeze(0)).float()
+
+    ref = {}
+
+    def keep(s, Q, rhoi):
+        if keep_all or s in marks or s == steps:
+            ref[s] = TY(Q, rhoi)
+    log(f"[bench] accuracy: FP64 plain reference, {steps} steps")
+    run("fp64", False, keep)
+    out = {}
+    for name, prec, nn, pert in (("fp32_newton", "fp32", False, 0.0), ("fp64_nn", "fp64", True, 0.0),
+                                 ("fp32_nn", "fp32", True, 0.0),
+                                 ("fp64_perturbed", "fp64", False, 1e-7)):
+        curve = {"step": [], "T_linf_K": [], "Y_linf": [], "cells_T_over_1K": []}
+
+        def cmp(s, Q, rhoi):
+            if s not in ref:
+                return
+            T, Y = TY(Q, rhoi)
+            Tr, Yr = ref[s]
+            dT = (T - Tr).abs()
+            curve["step"].append(s)
+            curve["T_linf_K"].append(float(dT.max()))
+            curve["Y_linf"].append(float((Y - Yr).abs().max()))
+            curve["cells_T_over_1K"].append(int((dT > 1.0).sum()))
+        log(f"[bench] accuracy: {name}")
+        run(prec, nn, cmp, pert)
+        first = next((s for s, t, y in zip(curve["step"], curve["T_linf_K"], curve["Y_linf"])
+                      if t > ACC_BAR["T_K"] or y > ACC_BAR["Y"]), None)
+        at = {str(s): {"T_linf_K": curve["T_linf_K"][i], "Y_linf": curve["Y_linf"][i],
+                       "cells_T_over_1K": curve["cells_T_over_1K"][i]}
+              for i, s in enumerate(curve["step"]) if s in marks or s == steps}
+        out[name] = {"at_step": at, "first_step_over_bar": first, "curve": curve}
+    del ref
+    torch.cuda.empty_cache()
+    return {"bar": ACC_BAR, "steps": steps, "vs": "FP64 plain (FP64 stage + FP64 Newton chemistry)",
+            **out}
+
+
 def cpu_baseline(mech, size, steps, min_seconds=10.0):
     """The FP64 oracle (OpenMP over the host cores) on a bounded sample of the
     same workload: a size^3 block of the reacting mixing layer, whole time

@@ -52,12 +52,13 @@ class Scheme(C.Structure):
     _fields_ = [("SWeps2", C.c_double), ("WENOeps", C.c_double), ("hybrid_phi1", C.c_double),
                 ("hybrid_phi2", C.c_double), ("UP7", C.c_double * 7), ("T_criteria", C.c_double),
                 ("precision", C.c_int32), ("bug_compat", C.c_int32), ("sensor_kind", C.c_int32),
-                ("recon_kind", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("recon_kind", C.c_int32), ("stage_kind", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 NN_NPARAM = 12512  # F3D_NN_NPARAM
 SENSOR_PRESSURE, SENSOR_DUCROS = 0, 1  # F3D_SENSOR_*
 RECON_REFERENCE, RECON_WENO7 = 0, 1    # F3D_RECON_*
+STAGE_FUSED, STAGE_GRANULAR = 0, 1     # F3D_STAGE_*
 # Ducros switch (UP7 below theta = 0.1, WENO5-Z above, never NND)
 DUCROS_SCHEME = dict(sensor_kind=SENSOR_DUCROS, hybrid_phi1=0.1, hybrid_phi2=2.0)
 
@@ -117,6 +118,10 @@ _PROTOS = {
     "f3d_time_step": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                       C.c_double, C.c_double, C.POINTER(C.c_int64), _vp, _vp],
     "f3d_nan_check": [_vp, _vp, C.POINTER(C.c_int32), _vp],
+    "f3d_nn_timing": [_vp, C.c_int32, C.POINTER(C.c_double)],
+    "f3d_sweep_conv": [_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp],
+    "f3d_sweep_diff": [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "f3d_sweep_stage_end": [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp],
     "f3d_last_error": [],
 }
 
@@ -179,7 +184,8 @@ class Context:
         self.precision = precision
         self.rdtype = torch.float64 if precision == "fp64" else torch.float32
         s = dict(SWeps2=0.0, WENOeps=1e-26, hybrid_phi1=1e-3, hybrid_phi2=0.1, UP7=UP7,
-                 T_criteria=500.0, sensor_kind=SENSOR_PRESSURE, recon_kind=RECON_REFERENCE)
+                 T_criteria=500.0, sensor_kind=SENSOR_PRESSURE, recon_kind=RECON_REFERENCE,
+                 stage_kind=STAGE_FUSED)
         if scheme:
             s.update(scheme)
         m = mech
@@ -196,7 +202,7 @@ class Context:
         sc = Scheme(s["SWeps2"], s["WENOeps"], s["hybrid_phi1"], s["hybrid_phi2"],
                     (C.c_double * 7)(*s["UP7"]), s["T_criteria"],
                     F3D_FP64 if precision == "fp64" else F3D_FP32, bug_compat,
-                    int(s["sensor_kind"]), int(s["recon_kind"]))
+                    int(s["sensor_kind"]), int(s["recon_kind"]), int(s["stage_kind"]))
         dims = Dims(Nx, Ny, Nz, NG, m.Ns, m.Nr)
         h = _vp()
         _rc(self.lib.f3d_create(C.byref(dims), C.byref(th), C.byref(kin), C.byref(sc),
@@ -482,6 +488,45 @@ class Context:
                                    self._pad(Q, 7, "Q"), dt, chem, sub_step, maxiter, rtol, ttol,
                                    C.byref(nfb), self._tag(tag), self.stream()), "f3d_time_step")
         return int(nfb.value)
+
+    # ---- fused float stage pieces (F3D_STAGE_FUSED) ---------------------
+    def rhs_zeros(self):
+        """The fused stage's right-hand side [5 + Ns][Nz][Ny][Nx], float."""
+        return self.torch.zeros((5 + self.Ns, self.Nz, self.Ny, self.Nx),
+                                dtype=self.torch.float32, device="cuda")
+
+    def _rhs(self, rhs):
+        return self._chk(rhs, (5 + self.Ns, self.Nz, self.Ny, self.Nx), self.torch.float32, "rhs")
+
+    def sweep_conv(self, d, rhs, Q, rhoi, phi, accumulate):
+        _rc(self.lib.f3d_sweep_conv(self.h, d, self._rhs(rhs), self._pad(Q, 7, "Q"),
+                                    self._pad(rhoi, self.Ns, "rhoi"),
+                                    self._rpad(phi, None, "phi"), int(bool(accumulate)),
+                                    self.stream()), "f3d_sweep_conv")
+
+    def sweep_diff(self, d, rhs, Q, Yi, lam, mu, D):
+        _rc(self.lib.f3d_sweep_diff(self.h, d, self._rhs(rhs), self._pad(Q, 7, "Q"),
+                                    self._rpad(Yi, self.Ns, "Yi"), self._rpad(lam, None, "lambda"),
+                                    self._rpad(mu, None, "mu"), self._rpad(D, self.Ns, "D"),
+                                    self.stream()), "f3d_sweep_diff")
+
+    def sweep_stage_end(self, KRK, U, Un, rhoi, rhon, Q, rhs, dt):
+        Ns = self.Ns
+        _rc(self.lib.f3d_sweep_stage_end(self.h, KRK, self._pad(U, 5, "U"), self._pad(Un, 5, "Un"),
+                                         self._pad(rhoi, Ns, "rhoi"), self._pad(rhon, Ns, "rhon"),
+                                         self._pad(Q, 7, "Q"), self._rhs(rhs), dt, self.stream()),
+            "f3d_sweep_stage_end")
+
+    def nn_timing(self, enable=None):
+        """Enable (True) / disable (False) event timing of f3d_nn_chem; with
+        enable None return the last timed call's [network kernel, scan +
+        compaction, read-back + Newton fallback] in ms."""
+        if enable is not None:
+            _rc(self.lib.f3d_nn_timing(self.h, int(bool(enable)), None), "f3d_nn_timing")
+            return None
+        ms = (C.c_double * 3)()
+        _rc(self.lib.f3d_nn_timing(self.h, 1, ms), "f3d_nn_timing")
+        return list(ms)
 
     def nan_check(self, U) -> bool:
         flag = C.c_int32(0)

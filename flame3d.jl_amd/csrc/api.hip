@@ -114,11 +114,12 @@ static size_t real_size(const f3d_ctx* c) { return c->precision == F3D_FP64 ? 8 
 enum WsId {
   W_Yi, W_lam, W_mu, W_D, W_phi, W_Fp, W_Fm, W_Fpi, W_Fmi,
   W_Fx, W_Fy, W_Fz, W_Fvx, W_Fvy, W_Fvz, W_Fxi, W_Fyi, W_Fzi,
-  W_Fdx, W_Fdy, W_Fdz, W_Fhx, W_Fhy, W_Fhz, W_COUNT
+  W_Fdx, W_Fdy, W_Fdz, W_Fhx, W_Fhy, W_Fhz, W_RHS, W_COUNT
 };
 static const char* kWsNames[W_COUNT] = {
     "Yi", "lambda", "mu", "D", "phi", "Fp", "Fm", "Fp_i", "Fm_i", "Fx", "Fy", "Fz",
-    "Fvx", "Fvy", "Fvz", "Fx_i", "Fy_i", "Fz_i", "Fdx", "Fdy", "Fdz", "Fhx", "Fhy", "Fhz"};
+    "Fvx", "Fvy", "Fvz", "Fx_i", "Fy_i", "Fz_i", "Fdx", "Fdy", "Fdz", "Fhx", "Fhy", "Fhz",
+    "rhs"};
 
 static int ensure_workspace(f3d_ctx* c) {
   if (c->ws_base) return F3D_OK;
@@ -141,6 +142,8 @@ static int ensure_workspace(f3d_ctx* c) {
     elems[W_Fdx + d] = nf[d] * Ns;
     elems[W_Fhx + d] = nf[d] * 3;
   }
+  // the fused FP32 stage's right-hand side [5 + Ns][Nx*Ny*Nz]
+  elems[W_RHS] = c->precision == F3D_FP32 ? (size_t)g.Nx * g.Ny * g.Nz * (5 + Ns) : 0;
   const size_t rs = real_size(c);
   size_t off[W_COUNT], total = 0;
   for (int w = 0; w < W_COUNT; ++w) {
@@ -215,6 +218,10 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
     set_error("f3d_create: recon_kind must be F3D_RECON_REFERENCE or F3D_RECON_WENO7");
     return F3D_ERR_ARG;
   }
+  if (sc->stage_kind != F3D_STAGE_FUSED && sc->stage_kind != F3D_STAGE_GRANULAR) {
+    set_error("f3d_create: stage_kind must be F3D_STAGE_FUSED or F3D_STAGE_GRANULAR");
+    return F3D_ERR_ARG;
+  }
   if (sc->sensor_kind != F3D_SENSOR_PRESSURE && sc->sensor_kind != F3D_SENSOR_DUCROS) {
     set_error("f3d_create: sensor_kind must be F3D_SENSOR_PRESSURE or F3D_SENSOR_DUCROS");
     return F3D_ERR_ARG;
@@ -236,6 +243,7 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
   c->precision = sc->precision;
   c->bug_compat = sc->bug_compat;
   c->sensor_kind = sc->sensor_kind;
+  c->stage_kind = sc->stage_kind;
   int rc = fill_params(c->host_params, d, th, kin, sc);
   if (rc) {
     delete c;
@@ -579,6 +587,62 @@ int f3d_set_metrics(f3d_ctx* c, const double* const* metrics, const double* J) {
   return F3D_OK;
 }
 
+static int sweep_ready(f3d_ctx* c, const char* what) {
+  if (c->precision != F3D_FP32 || !c->mf[0]) {
+    set_error(std::string(what) + ": needs an F3D_FP32 context after f3d_set_metrics");
+    return F3D_ERR_STATE;
+  }
+  return F3D_OK;
+}
+
+int f3d_sweep_conv(f3d_ctx* c, int32_t dir, float* rhs, const double* Q, const double* rhoi,
+                   const float* phi, int32_t accumulate, hipStream_t s) {
+  CHECK_CTX(c);
+  CHECK_PTR(rhs, "rhs");
+  CHECK_PTR(Q, "Q");
+  CHECK_PTR(rhoi, "rhoi");
+  CHECK_PTR(phi, "phi");
+  if (dir < 0 || dir > 2) {
+    set_error("f3d_sweep_conv: dir must be 0, 1 or 2");
+    return F3D_ERR_ARG;
+  }
+  RET(sweep_ready(c, "f3d_sweep_conv"));
+  return launch_sweep_conv(c, dir, rhs, Q, rhoi, phi, accumulate != 0, s);
+}
+
+int f3d_sweep_diff(f3d_ctx* c, int32_t dir, float* rhs, const double* Q, const float* Yi,
+                   const float* lambda, const float* mu, const float* D, hipStream_t s) {
+  CHECK_CTX(c);
+  CHECK_PTR(rhs, "rhs");
+  CHECK_PTR(Q, "Q");
+  CHECK_PTR(Yi, "Yi");
+  CHECK_PTR(lambda, "lambda");
+  CHECK_PTR(mu, "mu");
+  CHECK_PTR(D, "D");
+  if (dir < 0 || dir > 2) {
+    set_error("f3d_sweep_diff: dir must be 0, 1 or 2");
+    return F3D_ERR_ARG;
+  }
+  RET(sweep_ready(c, "f3d_sweep_diff"));
+  return launch_sweep_diff(c, dir, rhs, Q, Yi, lambda, mu, D, s);
+}
+
+int f3d_sweep_stage_end(f3d_ctx* c, int32_t KRK, double* U, const double* Un, double* rhoi,
+                        const double* rhon, double* Q, const float* rhs, double dt,
+                        hipStream_t s) {
+  CHECK_CTX(c);
+  CHECK_PTR(U, "U");
+  CHECK_PTR(rhoi, "rhoi");
+  CHECK_PTR(Q, "Q");
+  CHECK_PTR(rhs, "rhs");
+  if (KRK < 1 || KRK > 3 || (KRK > 1 && (!Un || !rhon))) {
+    set_error("f3d_sweep_stage_end: KRK must be 1, 2 or 3 (2 and 3 need Un, rhon)");
+    return F3D_ERR_ARG;
+  }
+  RET(sweep_ready(c, "f3d_sweep_stage_end"));
+  return launch_sweep_stage_end(c, KRK, U, Un, rhoi, rhon, Q, rhs, dt, c->J, s);
+}
+
 void* f3d_workspace(f3d_ctx* c, const char* name) {
   if (!c || !name || !c->ws_base) return nullptr;
   for (int w = 0; w < W_COUNT; ++w)
@@ -619,6 +683,14 @@ int f3d_rk3_stage(f3d_ctx* c, int32_t KRK, double* U, double* Un, double* rhoi, 
   }
   RET(launch_mixture(c, Q, rhoi, w[W_Yi], w[W_lam], w[W_mu], w[W_D], tag, s));
   RET(launch_shock_sensor(c, w[W_phi], Q, s));
+  if (mf && !tag && c->stage_kind == F3D_STAGE_FUSED) {
+    // fused FP32 stage: per-direction sweeps into one right-hand side, then
+    // divSpecs + div + linComb + c2Prim in one pass
+    float* rhs = (float*)w[W_RHS];
+    RET(launch_sweeps(c, rhs, Q, rhoi, w[W_phi], w[W_Yi], w[W_lam], w[W_mu], w[W_D], s));
+    RET(launch_sweep_stage_end(c, KRK, U, Un, rhoi, rhon, Q, rhs, dt, J, s));
+    return launch_fill(c, Q, U, rhoi, s);
+  }
   // specAdvance, solver.jl:40-56
   for (int d = 0; d < 3; ++d) {
     RET(launch_species_split(c, rhoi, Q, w[W_Fpi], w[W_Fmi], M[3 * d], M[3 * d + 1],

@@ -54,6 +54,7 @@ struct f3d_ctx {
   int precision;
   int bug_compat;
   int sensor_kind = 0;  // F3D_SENSOR_PRESSURE / F3D_SENSOR_DUCROS
+  int stage_kind = 0;   // F3D_STAGE_FUSED / F3D_STAGE_GRANULAR (F3D_FP32 contexts)
   f3d::DevParams host_params;
   f3d::DevParams* dparams = nullptr;
   // boundary fills
@@ -88,6 +89,11 @@ struct f3d_ctx {
   int32_t* nn_fbcount = nullptr;  // rejected cells per 64-cell tile, then their offsets
   int32_t* nn_fblist = nullptr;   // rejected cell ids in cell order [Nx*Ny*Nz]
   int32_t* nn_fbtotal = nullptr;  // number of rejected cells
+  // f3d_nn_timing: events around the last f3d_nn_chem's network kernel, its
+  // scan / compaction and its Newton fallback, on the call's stream
+  bool nn_timing = false;
+  bool nn_timed = false;
+  hipEvent_t nn_ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace f3d {
@@ -368,6 +374,18 @@ int launch_stage_end(f3d_ctx* c, int KRK, double* U, const double* Un, double* r
                      const double* rhon, double* Q, void* const* F, double dt, const double* J,
                      const int64_t* tag, hipStream_t s);
 int launch_prim2c(f3d_ctx* c, double* U, const double* Q, hipStream_t s);
+// fused FP32 sweeps (sweep.hip): every direction's convective and diffusive
+// face-flux differences into rhs[5 + Ns][Nx*Ny*Nz], then the stage end
+int launch_sweep_conv(f3d_ctx* c, int dir, float* rhs, const double* Q, const double* rhoi,
+                      const void* phi, int acc, hipStream_t s);
+int launch_sweep_diff(f3d_ctx* c, int dir, float* rhs, const double* Q, const void* Yi,
+                      const void* lam, const void* mu, const void* D, hipStream_t s);
+int launch_sweeps(f3d_ctx* c, float* rhs, const double* Q, const double* rhoi, const void* phi,
+                  const void* Yi, const void* lam, const void* mu, const void* D,
+                  hipStream_t s);
+int launch_sweep_stage_end(f3d_ctx* c, int KRK, double* U, const double* Un, double* rhoi,
+                           const double* rhon, double* Q, const float* rhs, double dt,
+                           const double* J, hipStream_t s);
 int launch_lincomb(f3d_ctx* c, double* U, const double* Un, int NV, double a, double b,
                    const int64_t* tag, hipStream_t s);
 int launch_get_y(f3d_ctx* c, double* Yi, const double* rhoi, const double* Q,

@@ -845,6 +845,12 @@ void nn_free(f3d_ctx* c) {
                   c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (hipEvent_t& e : c->nn_ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  c->nn_timing = c->nn_timed = false;
   c->nn_ready = false;
 }
 
@@ -1071,6 +1077,8 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   const long ncell = (long)c->g.Nx * c->g.Ny * c->g.Nz;
   const long ntile = (ncell + 63) / 64;
   int8_t* st = status ? status : c->nn_status;
+  const bool timed = c->nn_timing;
+  if (timed) (void)hipEventRecord(c->nn_ev[0], s);
   // every tile writes its count, so no clearing is needed
   if (c->g.Ns == 9)
     hipLaunchKernelGGL((k_nn_chem<9>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
@@ -1080,6 +1088,7 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
     hipLaunchKernelGGL((k_nn_chem<5>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
                        c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
                        c->g, dt, c->nn_fbcount, st);
+  if (timed) (void)hipEventRecord(c->nn_ev[1], s);
   // tile counts -> tile offsets; the chunk sums sit after the tile counts
   const long nchunk = (ntile + kScanChunk - 1) / kScanChunk;
   int32_t* sums = c->nn_fbcount + ntile;
@@ -1092,6 +1101,7 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
                      ncell, c->nn_fblist);
   rc = hip_check(hipGetLastError(), "f3d_nn_chem");
   if (rc) return rc;
+  if (timed) (void)hipEventRecord(c->nn_ev[2], s);
   int32_t nfb = 0;
   rc = hip_check(hipMemcpyAsync(&nfb, c->nn_fbtotal, sizeof(int32_t), hipMemcpyDeviceToHost, s),
                  "f3d_nn_chem");
@@ -1101,9 +1111,47 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   *n_fallback = nfb;
   // the rejected cells, in cell order: the stiff ones stay grouped in a few
   // waves (a wave's cost is its slowest cell) instead of one in every wave
-  if (nfb > 0)
-    return launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fblist,
-                       nfb);
+  if (timed) (void)hipEventRecord(c->nn_ev[3], s);  // after the read-back
+  if (nfb > 0) {
+    rc = launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fblist,
+                     nfb);
+    if (rc) return rc;
+  }
+  if (timed) {
+    // the fallback's end replaces the read-back mark: ev[3] - ev[2] then spans
+    // the 4-byte read-back and the Newton fallback launch
+    (void)hipEventRecord(c->nn_ev[3], s);
+    c->nn_timed = true;
+  }
+  return F3D_OK;
+}
+
+int f3d_nn_timing(f3d_ctx* c, int32_t enable, double* ms) {
+  if (!c) {
+    set_error("f3d_nn_timing: null context");
+    return F3D_ERR_ARG;
+  }
+  if (enable && !c->nn_ev[0]) {
+    for (hipEvent_t& e : c->nn_ev) {
+      int rc = hip_check(hipEventCreate(&e), "f3d_nn_timing");
+      if (rc) return rc;
+    }
+  }
+  c->nn_timing = enable != 0;
+  if (ms) {
+    if (!c->nn_timed) {
+      set_error("f3d_nn_timing: no timed f3d_nn_chem call yet");
+      return F3D_ERR_STATE;
+    }
+    int rc = hip_check(hipEventSynchronize(c->nn_ev[3]), "f3d_nn_timing");
+    if (rc) return rc;
+    float t[3];
+    for (int k = 0; k < 3; ++k) {
+      rc = hip_check(hipEventElapsedTime(&t[k], c->nn_ev[k], c->nn_ev[k + 1]), "f3d_nn_timing");
+      if (rc) return rc;
+      ms[k] = t[k];
+    }
+  }
   return F3D_OK;
 }
 

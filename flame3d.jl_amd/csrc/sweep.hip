@@ -1,0 +1,850 @@
+// Fused FP32 stage sweeps.  Per direction, the convective right-hand side
+// (Steger-Warming splitting -> hybrid UP7 / WENO5-Z / NND reconstruction ->
+// face-flux difference, split.jl / schemes.jl) and the diffusive one
+// (viscous stresses and heat flux, species diffusion and its enthalpy flux,
+// viscous.jl) of every interior cell are accumulated into one float array
+//   rhs[v][cell],  v = 0..4 (rho, rho u, rho v, rho w, E), 5 + k (rho_k),
+//   cell = a + Nx (b + Ny c),
+// so the split fluxes, face fluxes, Fv, Fd and Fh of the reference's
+// decomposition (solver.jl:22-56) never reach HBM; the stage end applies
+// U += J dt rhs (div.jl:16-84), linComb and c2Prim in one pass.
+//
+// x sweeps: one 256-thread block per segment of up to 256 cells of an x line;
+// the split fluxes (convective) or the face values (diffusive) of the segment
+// are staged in LDS so each point is read from HBM once and each face is
+// evaluated once; the face-flux difference of a cell reads its two faces
+// from LDS.  y and z sweeps: one thread per (x, transverse) line marches
+// through a segment of cells with the stencil window in registers (x
+// coalesced loads), the previous face's value kept for the difference.
+// The convective march splits the variables over the waves of a block
+// (wave 0: the five flow fluxes, waves 1..: groups of three species) so each
+// wave's window fits its registers; the waves of a block march the same
+// lines and share their loads of u, v, w and the metrics through L1.
+//
+// Arithmetic per face is the kernel-granular path's (split_point,
+// weno_face_v, the viscous face formulas); the sums of the face differences
+// over the three directions are associated per direction instead of per
+// cell, a rounding-level difference on the float path.
+#include "f3d_stencil.h"
+
+namespace f3d {
+
+constexpr int kXSeg = 256;  // cells per x-line block
+constexpr int kConvSeg = 32;  // cells per march segment (convective)
+constexpr int kDiffSeg = 64;  // cells per march segment (diffusive)
+
+__device__ inline void rhs_put(float* __restrict__ r, long i, float d, int acc) {
+  r[i] = acc ? r[i] + d : d;
+}
+
+// ---------------------------------------------------------------------------
+// Convective x sweep
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ void __launch_bounds__(kXSeg)
+    k_sw_conv_x(float* __restrict__ rhs, const double* __restrict__ Q,
+                const double* __restrict__ rhoi, const float* __restrict__ A1,
+                const float* __restrict__ A2, const float* __restrict__ A3,
+                const float* __restrict__ phi, const DevParams* __restrict__ P, Geom g,
+                int acc) {
+  constexpr int NV = 5 + NS;
+  constexpr int NPT = kXSeg + 8;
+  __shared__ float sp[NV][NPT];
+  __shared__ float sm[NV][NPT];
+  __shared__ float sphi[NPT];
+  const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
+  const unsigned L = xcd_block();
+  const int seg = (int)(L % (unsigned)nseg);
+  const int line = (int)(L / (unsigned)nseg);
+  const int b = line % g.Ny, c = line / g.Ny;
+  const int c0 = seg * kXSeg;
+  const int nc = min(kXSeg, g.Nx - c0);
+  const int t = threadIdx.x;
+  const long NP = g.NP;
+  // point q of the segment is padded i = c0 + q (cell c0 + q - NG)
+  const long row = (long)c0 + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  const float sw = PT<float>::sweps2(P);
+  for (int q = t; q < nc + 8; q += kXSeg) {
+    const long p = row + q;
+    const float u = (float)Q[p + NP], v = (float)Q[p + 2 * NP], w = (float)Q[p + 3 * NP];
+    const float a1 = A1[p], a2 = A2[p], a3 = A3[p];
+    float fp[5], fm[5];
+    split_point<float>((float)Q[p], u, v, w, (float)Q[p + 4 * NP], (float)Q[p + 6 * NP], a1, a2,
+                       a3, sw, fp, fm);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      sp[n][q] = fp[n];
+      sm[n][q] = fm[n];
+    }
+    const float un = a1 * u + a2 * v + a3 * w;
+    const float Ep = 0.5f * (un + fabsf(un));
+    const float Em = 0.5f * (un - fabsf(un));
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const float r = (float)rhoi[p + (long)n * NP];
+      sp[5 + n][q] = Ep * r;
+      sm[5 + n][q] = Em * r;
+    }
+    sphi[q] = phi[p];
+  }
+  __syncthreads();
+  // face f (0..nc) between cells c0+f-1 and c0+f: stencil points q = f .. f+7
+  float fv[NV];
+  if (t < nc) {
+    const int br = weno_branch<float>(fmaxf(sphi[t + 3], sphi[t + 4]), P);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      float pv[8], mv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        pv[k] = sp[n][t + k];
+        mv[k] = sm[n][t + k];
+      }
+      fv[n] = weno_face_v<float>(pv, mv, br, P);
+    }
+  }
+  // the segment's last face, one variable per thread
+  float fl = 0.0f;
+  if (t < NV) {
+    const int br = weno_branch<float>(fmaxf(sphi[nc + 3], sphi[nc + 4]), P);
+    float pv[8], mv[8];
+    for (int k = 0; k < 8; ++k) {
+      pv[k] = sp[t][nc + k];
+      mv[k] = sm[t][nc + k];
+    }
+    fl = weno_face_v<float>(pv, mv, br, P);
+  }
+  __syncthreads();
+  float(*sf)[NPT] = sp;  // face values, reusing the F+ tile
+  if (t < nc) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) sf[n][t] = fv[n];
+  }
+  if (t < NV) sf[t][nc] = fl;
+  __syncthreads();
+  if (t < nc) {
+    const long NC = (long)g.Nx * g.Ny * g.Nz;
+    const long cell = (c0 + t) + (long)g.Nx * (b + (long)g.Ny * c);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) rhs_put(rhs, n * NC + cell, sf[n][t] - sf[n][t + 1], acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Convective y / z march
+// ---------------------------------------------------------------------------
+struct SweepLine {
+  int a, tr, c0, c1;  // x index, other transverse index, cells [c0, c1) along the sweep
+  long base, s;       // padded index of (a, tr) at row -NG; sweep stride
+  long cell0, cstr;   // rhs cell index of row 0 and its stride along the sweep
+};
+
+__device__ inline bool sweep_line(const Geom& g, int dir, int seg, SweepLine& m) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned hw = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned L = xcd_remap(hw, nb);
+  const int bx = (int)(L % gridDim.x);
+  m.tr = (int)((L / gridDim.x) % gridDim.y);
+  const int sg = (int)(L / (gridDim.x * gridDim.y));
+  m.a = bx * 64 + (int)threadIdx.x;
+  if (m.a >= g.Nx) return false;
+  const int nd = dir == 1 ? g.Ny : g.Nz;
+  m.c0 = sg * seg;
+  m.c1 = min(m.c0 + seg, nd);
+  if (m.c0 >= m.c1) return false;
+  const long sy = g.Nxt, sz = (long)g.Nxt * g.Nyt;
+  m.s = (dir == 1) ? sy : sz;
+  const long t2 = (dir == 1) ? sz : sy;
+  m.base = (m.a + NG) + t2 * (m.tr + NG);
+  m.cell0 = (dir == 1) ? m.a + (long)g.Nx * g.Ny * m.tr : m.a + (long)g.Nx * m.tr;
+  m.cstr = (dir == 1) ? (long)g.Nx : (long)g.Nx * g.Ny;
+  return true;
+}
+
+// wave 0 of a block: the five flow fluxes
+__device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __restrict__ Q,
+                                       const float* __restrict__ A1, const float* __restrict__ A2,
+                                       const float* __restrict__ A3,
+                                       const float* __restrict__ phi,
+                                       const DevParams* __restrict__ P, const Geom& g,
+                                       const SweepLine& m, int acc) {
+  const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
+  const float sw = PT<float>::sweps2(P);
+  float wp[5][8], wm[5][8];
+  auto load = [&](int r, int slot) {
+    const long p = m.base + (long)(r + NG) * m.s;
+    float fp[5], fm[5];
+    split_point<float>((float)Q[p], (float)Q[p + NP], (float)Q[p + 2 * NP], (float)Q[p + 3 * NP],
+                       (float)Q[p + 4 * NP], (float)Q[p + 6 * NP], A1[p], A2[p], A3[p], sw, fp,
+                       fm);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      wp[n][slot] = fp[n];
+      wm[n][slot] = fm[n];
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 7; ++k) load(m.c0 - 4 + k, k);
+  float phA = phi[m.base + (long)(m.c0 - 1 + NG) * m.s];
+  float prev[5];
+  for (int f = m.c0; f <= m.c1; ++f) {
+    load(f + 3, 7);
+    const float phB = phi[m.base + (long)(f + NG) * m.s];
+    const int br = weno_branch<float>(fmaxf(phA, phB), P);
+    float cur[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) cur[n] = weno_face_v<float>(wp[n], wm[n], br, P);
+    if (f > m.c0) {
+      const long cell = m.cell0 + (long)(f - 1) * m.cstr;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) rhs_put(rhs, n * NC + cell, prev[n] - cur[n], acc);
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      prev[n] = cur[n];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        wp[n][k] = wp[n][k + 1];
+        wm[n][k] = wm[n][k + 1];
+      }
+    }
+    phA = phB;
+  }
+}
+
+// waves 1..: species k0 .. k0+2
+template <int NS>
+__device__ inline void conv_march_species(float* __restrict__ rhs, const double* __restrict__ Q,
+                                          const double* __restrict__ rhoi,
+                                          const float* __restrict__ A1,
+                                          const float* __restrict__ A2,
+                                          const float* __restrict__ A3,
+                                          const float* __restrict__ phi,
+                                          const DevParams* __restrict__ P, const Geom& g,
+                                          const SweepLine& m, int k0, int acc) {
+  const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
+  float wp[3][8], wm[3][8];
+  auto load = [&](int r, int slot) {
+    const long p = m.base + (long)(r + NG) * m.s;
+    const float un =
+        A1[p] * (float)Q[p + NP] + A2[p] * (float)Q[p + 2 * NP] + A3[p] * (float)Q[p + 3 * NP];
+    const float Ep = 0.5f * (un + fabsf(un));
+    const float Em = 0.5f * (un - fabsf(un));
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      const float r = (k0 + kk < NS) ? (float)rhoi[p + (long)(k0 + kk) * NP] : 0.0f;
+      wp[kk][slot] = Ep * r;
+      wm[kk][slot] = Em * r;
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 7; ++k) load(m.c0 - 4 + k, k);
+  float phA = phi[m.base + (long)(m.c0 - 1 + NG) * m.s];
+  float prev[3];
+  for (int f = m.c0; f <= m.c1; ++f) {
+    load(f + 3, 7);
+    const float phB = phi[m.base + (long)(f + NG) * m.s];
+    const int br = weno_branch<float>(fmaxf(phA, phB), P);
+    float cur[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      cur[kk] = (k0 + kk < NS) ? weno_face_v<float>(wp[kk], wm[kk], br, P) : 0.0f;
+    if (f > m.c0) {
+      const long cell = m.cell0 + (long)(f - 1) * m.cstr;
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk)
+        if (k0 + kk < NS) rhs_put(rhs, (5 + k0 + kk) * NC + cell, prev[kk] - cur[kk], acc);
+    }
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      prev[kk] = cur[kk];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        wp[kk][k] = wp[kk][k + 1];
+        wm[kk][k] = wm[kk][k + 1];
+      }
+    }
+    phA = phB;
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(64 * (1 + (NS + 2) / 3))
+    k_sw_conv_m(float* __restrict__ rhs, const double* __restrict__ Q,
+                const double* __restrict__ rhoi, const float* __restrict__ A1,
+                const float* __restrict__ A2, const float* __restrict__ A3,
+                const float* __restrict__ phi, const DevParams* __restrict__ P, Geom g, int dir,
+                int seg, int acc) {
+  SweepLine m;
+  if (!sweep_line(g, dir, seg, m)) return;
+  const int grp = threadIdx.y;
+  if (grp == 0)
+    conv_march_flow(rhs, Q, A1, A2, A3, phi, P, g, m, acc);
+  else
+    conv_march_species<NS>(rhs, Q, rhoi, A1, A2, A3, phi, P, g, m, 3 * (grp - 1), acc);
+}
+
+// ---------------------------------------------------------------------------
+// Diffusive x sweep: viscous stresses and heat flux (viscous.jl:1-142) and
+// species diffusion (viscous.jl:430-507) at one face, float arithmetic as the
+// kernel-granular k_viscous / k_spec_viscous; the enthalpy flux Fh enters the
+// energy flux through the face normal, n . Fh.
+// ---------------------------------------------------------------------------
+struct FMet {
+  const float* m[9];
+};
+
+template <int NS>
+__device__ inline void spec_face(const double* __restrict__ Q, const float* __restrict__ Yi,
+                                 const FMet& M, const float* __restrict__ J,
+                                 const float* __restrict__ D, const DevParams* __restrict__ P,
+                                 const Geom& g, long ic, int dir, float* out) {
+  const long st[3] = {1, (long)g.Nxt, (long)g.Nxt * g.Nyt};
+  const long s = st[dir];
+  const long im = ic - s;
+  const long NP = g.NP;
+  float m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = (M.m[q][im] + M.m[q][ic]) * 0.5f;
+  const float Jac = (J[im] + J[ic]) * 0.5f;
+  const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
+  const float c12 = 1.0f / 12.0f;
+  const float rho = ((float)Q[im] + (float)Q[ic]) * 0.5f;
+  const float T = ((float)Q[im + 5 * NP] + (float)Q[ic + 5 * NP]) * 0.5f;
+  float hi[NS], Vk1[NS], Vk2[NS], Vk3[NS];
+  h_specs<NS, float>(P, T, hi);
+  float sum1 = 0, sum2 = 0, sum3 = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const float* y = Yi + (long)n * NP;
+    const float rhoDi = (D[im + (long)n * NP] + D[ic + (long)n * NP]) * rho;
+    float dY[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+      dY[t] = (t == dir) ? 1.25f * (y[ic] - y[im]) - c12 * (y[ic + s] - y[ic - 2 * s])
+                         : 0.25f * (y[ic + st[t]] + y[im + st[t]] - y[ic - st[t]] - y[im - st[t]]);
+    const float Vx = (dY[0] * m[0] + dY[1] * m[3] + dY[2] * m[6]) * Jac * rhoDi;
+    const float Vy = (dY[0] * m[1] + dY[1] * m[4] + dY[2] * m[7]) * Jac * rhoDi;
+    const float Vz = (dY[0] * m[2] + dY[1] * m[5] + dY[2] * m[8]) * Jac * rhoDi;
+    Vk1[n] = Vx;
+    Vk2[n] = Vy;
+    Vk3[n] = Vz;
+    sum1 += Vx;
+    sum2 += Vy;
+    sum3 += Vz;
+  }
+  float fh1 = 0, fh2 = 0, fh3 = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const float Yn = 0.5f * (Yi[im + (long)n * NP] + Yi[ic + (long)n * NP]);
+    const float V1 = Vk1[n] - sum1 * Yn;
+    const float V2 = Vk2[n] - sum2 * Yn;
+    const float V3 = Vk3[n] - sum3 * Yn;
+    out[n] = V1 * n0 + V2 * n1 + V3 * n2;
+    fh1 += V1 * hi[n];
+    fh2 += V2 * hi[n];
+    fh3 += V3 * hi[n];
+  }
+  out[NS] = n0 * fh1 + n1 * fh2 + n2 * fh3;
+}
+
+__device__ inline void visc_face(const double* __restrict__ Q, const FMet& M,
+                                 const float* __restrict__ J, const float* __restrict__ lam,
+                                 const float* __restrict__ mu, const Geom& g, long ic, int dir,
+                                 float* out) {
+  const long st[3] = {1, (long)g.Nxt, (long)g.Nxt * g.Nyt};
+  const long s = st[dir];
+  const long im = ic - s;
+  const long NP = g.NP;
+  float m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = (M.m[q][im] + M.m[q][ic]) * 0.5f;
+  const float Jac = (J[im] + J[ic]) * 0.5f;
+  const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
+  const float c12 = 1.0f / 12.0f, c23 = 2.0f / 3.0f;
+  const float mui = (mu[im] + mu[ic]) * 0.5f;
+  const float lami = (lam[im] + lam[ic]) * 0.5f;
+  float dq[4][3];
+  const int qv[4] = {1, 2, 3, 5};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double* fq = Q + qv[q] * NP;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      if (t == dir) {
+        dq[q][t] = 1.25f * ((float)fq[ic] - (float)fq[im]) -
+                   c12 * ((float)fq[ic + s] - (float)fq[ic - 2 * s]);
+      } else {
+        const long tt = st[t];
+        dq[q][t] = 0.5f * (c23 * ((float)fq[ic + tt] + (float)fq[im + tt] - (float)fq[ic - tt] -
+                                  (float)fq[im - tt]) -
+                           c12 * ((float)fq[ic + 2 * tt] + (float)fq[im + 2 * tt] -
+                                  (float)fq[ic - 2 * tt] - (float)fq[im - 2 * tt]));
+      }
+    }
+  }
+  const float u = ((float)Q[im + NP] + (float)Q[ic + NP]) * 0.5f;
+  const float v = ((float)Q[im + 2 * NP] + (float)Q[ic + 2 * NP]) * 0.5f;
+  const float w = ((float)Q[im + 3 * NP] + (float)Q[ic + 3 * NP]) * 0.5f;
+  float gr[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      gr[q][j] = (dq[q][0] * m[j] + dq[q][1] * m[3 + j] + dq[q][2] * m[6 + j]) * Jac;
+  const float divu = gr[0][0] + gr[1][1] + gr[2][2];
+  const float t11 = mui * (2.0f * gr[0][0] - c23 * divu);
+  const float t12 = mui * (gr[0][1] + gr[1][0]);
+  const float t13 = mui * (gr[0][2] + gr[2][0]);
+  const float t22 = mui * (2.0f * gr[1][1] - c23 * divu);
+  const float t23 = mui * (gr[2][1] + gr[1][2]);
+  const float t33 = mui * (2.0f * gr[2][2] - c23 * divu);
+  const float E1 = u * t11 + v * t12 + w * t13 + lami * gr[3][0];
+  const float E2 = u * t12 + v * t22 + w * t23 + lami * gr[3][1];
+  const float E3 = u * t13 + v * t23 + w * t33 + lami * gr[3][2];
+  out[0] = n0 * t11 + n1 * t12 + n2 * t13;
+  out[1] = n0 * t12 + n1 * t22 + n2 * t23;
+  out[2] = n0 * t13 + n1 * t23 + n2 * t33;
+  out[3] = n0 * E1 + n1 * E2 + n2 * E3;
+}
+
+// x-line block of the diffusive sweeps: face values in LDS, then the
+// difference of each cell's two faces.  KIND 0: viscous (rhs 1..4), KIND 1:
+// species diffusion (rhs 5.. and the enthalpy flux into rhs 4).
+template <int KIND, int NS>
+__global__ void __launch_bounds__(kXSeg)
+    k_sw_diff_x(float* __restrict__ rhs, const double* __restrict__ Q,
+                const float* __restrict__ Yi, FMet M, const float* __restrict__ J,
+                const float* __restrict__ lam, const float* __restrict__ mu,
+                const float* __restrict__ D, const DevParams* __restrict__ P, Geom g) {
+  constexpr int NO = KIND == 0 ? 4 : NS + 1;
+  __shared__ float sf[NO][kXSeg + 1];
+  const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
+  const unsigned L = xcd_block();
+  const int seg = (int)(L % (unsigned)nseg);
+  const int line = (int)(L / (unsigned)nseg);
+  const int b = line % g.Ny, c = line / g.Ny;
+  const int c0 = seg * kXSeg;
+  const int nc = min(kXSeg, g.Nx - c0);
+  const int t = threadIdx.x;
+  const long row = (long)(c0 + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  auto face = [&](int f) {
+    float out[NO];
+    if (KIND == 0)
+      visc_face(Q, M, J, lam, mu, g, row + f, 0, out);
+    else
+      spec_face<NS>(Q, Yi, M, J, D, P, g, row + f, 0, out);
+#pragma unroll
+    for (int n = 0; n < NO; ++n) sf[n][f] = out[n];
+  };
+  if (t < nc) face(t);            // the left face of cell c0 + t
+  if (t == nc - 1) face(nc);      // the segment's last face
+  __syncthreads();
+  if (t < nc) {
+    const long NC = (long)g.Nx * g.Ny * g.Nz;
+    const long cell = (c0 + t) + (long)g.Nx * (b + (long)g.Ny * c);
+    // diffusive fluxes enter with the opposite sign of the convective ones
+    if (KIND == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rhs_put(rhs, (q + 1) * NC + cell, sf[q][t + 1] - sf[q][t], 1);
+    } else {
+#pragma unroll
+      for (int n = 0; n < NS; ++n)
+        rhs_put(rhs, (5 + n) * NC + cell, sf[n][t + 1] - sf[n][t], 1);
+      rhs_put(rhs, 4 * NC + cell, sf[NS][t + 1] - sf[NS][t], 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diffusive y / z marches.  The cross derivatives at face j-1/2 average the
+// cell-centred central differences of cells j-1 and j (viscous.jl:36-52,
+// :476-479), so each row's tangential differences are computed once and kept
+// for the next face.
+// ---------------------------------------------------------------------------
+__device__ inline float cd4(const double* __restrict__ q, long ic, long t) {
+  const float c23 = 2.0f / 3.0f, c12 = 1.0f / 12.0f;
+  return c23 * ((float)q[ic + t] - (float)q[ic - t]) -
+         c12 * ((float)q[ic + 2 * t] - (float)q[ic - 2 * t]);
+}
+
+__global__ void __launch_bounds__(64)
+    k_sw_visc_m(float* __restrict__ rhs, const double* __restrict__ Q, FMet M,
+                const float* __restrict__ J, const float* __restrict__ lam,
+                const float* __restrict__ mu, Geom g, int dir, int seg) {
+  SweepLine mm;
+  if (!sweep_line(g, dir, seg, mm)) return;
+  const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
+  const long t2 = (dir == 1) ? (long)g.Nxt * g.Nyt : (long)g.Nxt;
+  const int qv[4] = {1, 2, 3, 5};  // u, v, w, T
+  float q[4][4];                    // rows f-2 .. f+1
+  float dX[4][2], dT[4][2];         // x and other-transverse differences at rows f-1, f
+  float mt[10][2], mu2[2], la2[2];
+  auto row_cross = [&](long ic, int r) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double* fq = Q + qv[k] * NP;
+      dX[k][r] = cd4(fq, ic, 1);
+      dT[k][r] = cd4(fq, ic, t2);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) mt[k][r] = M.m[k][ic];
+    mt[9][r] = J[ic];
+    mu2[r] = mu[ic];
+    la2[r] = lam[ic];
+  };
+  {
+    const long ic0 = mm.base + (long)(mm.c0 + NG) * mm.s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) q[k][r + 1] = (float)Q[qv[k] * NP + ic0 + (r - 2) * mm.s];
+    row_cross(ic0 - mm.s, 1);
+  }
+  const float c12 = 1.0f / 12.0f, c23 = 2.0f / 3.0f;
+  float prev[4];
+  for (int f = mm.c0; f <= mm.c1; ++f) {
+    const long ic = mm.base + (long)(f + NG) * mm.s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      q[k][0] = q[k][1];
+      q[k][1] = q[k][2];
+      q[k][2] = q[k][3];
+      q[k][3] = (float)Q[qv[k] * NP + ic + mm.s];
+      dX[k][0] = dX[k][1];
+      dT[k][0] = dT[k][1];
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) mt[k][0] = mt[k][1];
+    mu2[0] = mu2[1];
+    la2[0] = la2[1];
+    row_cross(ic, 1);
+    float m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = (mt[k][0] + mt[k][1]) * 0.5f;
+    const float Jac = (mt[9][0] + mt[9][1]) * 0.5f;
+    const float mui = (mu2[0] + mu2[1]) * 0.5f;
+    const float lami = (la2[0] + la2[1]) * 0.5f;
+    float dq[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float dn = 1.25f * (q[k][2] - q[k][1]) - c12 * (q[k][3] - q[k][0]);
+      const float dx = 0.5f * (dX[k][0] + dX[k][1]);
+      const float dt = 0.5f * (dT[k][0] + dT[k][1]);
+      dq[k][0] = dx;
+      dq[k][1] = (dir == 1) ? dn : dt;
+      dq[k][2] = (dir == 1) ? dt : dn;
+    }
+    const float u = (q[0][1] + q[0][2]) * 0.5f;
+    const float v = (q[1][1] + q[1][2]) * 0.5f;
+    const float w = (q[2][1] + q[2][2]) * 0.5f;
+    float gr[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+        gr[k][jj] = (dq[k][0] * m[jj] + dq[k][1] * m[3 + jj] + dq[k][2] * m[6 + jj]) * Jac;
+    const float divu = gr[0][0] + gr[1][1] + gr[2][2];
+    const float t11 = mui * (2.0f * gr[0][0] - c23 * divu);
+    const float t12 = mui * (gr[0][1] + gr[1][0]);
+    const float t13 = mui * (gr[0][2] + gr[2][0]);
+    const float t22 = mui * (2.0f * gr[1][1] - c23 * divu);
+    const float t23 = mui * (gr[2][1] + gr[1][2]);
+    const float t33 = mui * (2.0f * gr[2][2] - c23 * divu);
+    const float E1 = u * t11 + v * t12 + w * t13 + lami * gr[3][0];
+    const float E2 = u * t12 + v * t22 + w * t23 + lami * gr[3][1];
+    const float E3 = u * t13 + v * t23 + w * t33 + lami * gr[3][2];
+    const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
+    float cur[4];
+    cur[0] = n0 * t11 + n1 * t12 + n2 * t13;
+    cur[1] = n0 * t12 + n1 * t22 + n2 * t23;
+    cur[2] = n0 * t13 + n1 * t23 + n2 * t33;
+    cur[3] = n0 * E1 + n1 * E2 + n2 * E3;
+    if (f > mm.c0) {
+      const long cell = mm.cell0 + (long)(f - 1) * mm.cstr;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rhs_put(rhs, (k + 1) * NC + cell, cur[k] - prev[k], 1);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) prev[k] = cur[k];
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(64)
+    k_sw_spec_m(float* __restrict__ rhs, const double* __restrict__ Q,
+                const float* __restrict__ Yi, FMet M, const float* __restrict__ J,
+                const float* __restrict__ D, const DevParams* __restrict__ P, Geom g, int dir,
+                int seg) {
+  SweepLine mm;
+  if (!sweep_line(g, dir, seg, mm)) return;
+  const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
+  const long t2 = (dir == 1) ? (long)g.Nxt * g.Nyt : (long)g.Nxt;
+  float y[NS][4];  // rows f-2 .. f+1
+  float cX[NS][2], cT[NS][2], d2[NS][2];
+  float mt[10][2], rho2[2], T2[2];
+  auto row_cross = [&](long ic, int r) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const float* yy = Yi + (long)n * NP;
+      cX[n][r] = yy[ic + 1] - yy[ic - 1];
+      cT[n][r] = yy[ic + t2] - yy[ic - t2];
+      d2[n][r] = D[ic + (long)n * NP];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) mt[k][r] = M.m[k][ic];
+    mt[9][r] = J[ic];
+    rho2[r] = (float)Q[ic];
+    T2[r] = (float)Q[ic + 5 * NP];
+  };
+  {
+    const long ic0 = mm.base + (long)(mm.c0 + NG) * mm.s;
+#pragma unroll
+    for (int n = 0; n < NS; ++n)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) y[n][r + 1] = Yi[(long)n * NP + ic0 + (r - 2) * mm.s];
+    row_cross(ic0 - mm.s, 1);
+  }
+  const float c12 = 1.0f / 12.0f;
+  float prev[NS + 1];
+  for (int f = mm.c0; f <= mm.c1; ++f) {
+    const long ic = mm.base + (long)(f + NG) * mm.s;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      y[n][0] = y[n][1];
+      y[n][1] = y[n][2];
+      y[n][2] = y[n][3];
+      y[n][3] = Yi[(long)n * NP + ic + mm.s];
+      cX[n][0] = cX[n][1];
+      cT[n][0] = cT[n][1];
+      d2[n][0] = d2[n][1];
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) mt[k][0] = mt[k][1];
+    rho2[0] = rho2[1];
+    T2[0] = T2[1];
+    row_cross(ic, 1);
+    float m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = (mt[k][0] + mt[k][1]) * 0.5f;
+    const float Jac = (mt[9][0] + mt[9][1]) * 0.5f;
+    const float rho = (rho2[0] + rho2[1]) * 0.5f;
+    const float T = (T2[0] + T2[1]) * 0.5f;
+    float hi[NS], V1[NS], V2[NS], V3[NS];
+    h_specs<NS, float>(P, T, hi);
+    float sum1 = 0, sum2 = 0, sum3 = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const float rhoDi = (d2[n][0] + d2[n][1]) * rho;
+      const float dn = 1.25f * (y[n][2] - y[n][1]) - c12 * (y[n][3] - y[n][0]);
+      const float dx = 0.25f * (cX[n][0] + cX[n][1]);
+      const float dt = 0.25f * (cT[n][0] + cT[n][1]);
+      const float e0 = dx, e1 = (dir == 1) ? dn : dt, e2 = (dir == 1) ? dt : dn;
+      const float Vx = (e0 * m[0] + e1 * m[3] + e2 * m[6]) * Jac * rhoDi;
+      const float Vy = (e0 * m[1] + e1 * m[4] + e2 * m[7]) * Jac * rhoDi;
+      const float Vz = (e0 * m[2] + e1 * m[5] + e2 * m[8]) * Jac * rhoDi;
+      V1[n] = Vx;
+      V2[n] = Vy;
+      V3[n] = Vz;
+      sum1 += Vx;
+      sum2 += Vy;
+      sum3 += Vz;
+    }
+    const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
+    float cur[NS + 1];
+    float fh1 = 0, fh2 = 0, fh3 = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const float Yn = 0.5f * (y[n][1] + y[n][2]);
+      const float a1 = V1[n] - sum1 * Yn;
+      const float a2 = V2[n] - sum2 * Yn;
+      const float a3 = V3[n] - sum3 * Yn;
+      cur[n] = a1 * n0 + a2 * n1 + a3 * n2;
+      fh1 += a1 * hi[n];
+      fh2 += a2 * hi[n];
+      fh3 += a3 * hi[n];
+    }
+    cur[NS] = n0 * fh1 + n1 * fh2 + n2 * fh3;
+    if (f > mm.c0) {
+      const long cell = mm.cell0 + (long)(f - 1) * mm.cstr;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) rhs_put(rhs, (5 + n) * NC + cell, cur[n] - prev[n], 1);
+      rhs_put(rhs, 4 * NC + cell, cur[NS] - prev[NS], 1);
+    }
+#pragma unroll
+    for (int n = 0; n <= NS; ++n) prev[n] = cur[n];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage end from the accumulated right-hand side: U += J dt rhs, rho_k += J dt
+// rhs (div.jl:16-84), linComb with the stage-1 copies for stages 2 and 3
+// (utils.jl:95-112), c2Prim with the warm-started e -> T Newton (utils.jl:2-48).
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_stage_end(double* __restrict__ U, const double* __restrict__ Un,
+                   double* __restrict__ rhoi, const double* __restrict__ rhon,
+                   double* __restrict__ Q, const float* __restrict__ rhs, double dt,
+                   const double* __restrict__ J, const DevParams* __restrict__ P, Geom g,
+                   int lin, double la, double lb) {
+  const long NC = (long)g.Nx * g.Ny * g.Nz;
+  const long id = gid();
+  if (id >= NC) return;
+  const int a = (int)(id % g.Nx);
+  const int tt = (int)(id / g.Nx);
+  const int b = tt % g.Ny, c = tt / g.Ny;
+  const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  const long NP = g.NP;
+  const float Jact = (float)(J[p] * dt);
+  double r[NS], u[5];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r[n] = rhoi[p + n * NP] + (double)(rhs[(5 + n) * NC + id] * Jact);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) u[n] = U[p + n * NP] + (double)(rhs[n * NC + id] * Jact);
+  if (lin) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) u[n] = u[n] * la + Un[p + n * NP] * lb;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) r[n] = r[n] * la + rhon[p + n * NP] * lb;
+  }
+#pragma unroll
+  for (int n = 0; n < 5; ++n) U[p + n * NP] = u[n];
+  const double rho = fmax(u[0], kEps);
+  const double rinv = 1 / rho;
+  double sr = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    r[n] = fmax(r[n], 0.0);
+    sr += r[n];
+  }
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r[n] *= rho / sr;
+  const double uu = u[1] * rinv;
+  const double vv = u[2] * rinv;
+  const double ww = u[3] * rinv;
+  const double ei = fmax(u[4] - 0.5 * rho * (uu * uu + vv * vv + ww * ww), kEps);
+  const double T = fmax(get_T_from<NS>(P, ei, r, Q[p + 5 * NP]), kEps);
+  const double pr = fmax(pmixture<NS>(P, T, r), kEps);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = r[n];
+  Q[p] = rho;
+  Q[p + NP] = uu;
+  Q[p + 2 * NP] = vv;
+  Q[p + 3 * NP] = ww;
+  Q[p + 4 * NP] = pr;
+  Q[p + 5 * NP] = T;
+  Q[p + 6 * NP] = ei;
+}
+
+// ---- launchers --------------------------------------------------------------
+// Convective sweep of direction dir into rhs (acc = 0: the first sweep of the
+// stage stores, later sweeps add).  A[3]: the sweep's metrics (float copies).
+template <int NS>
+static void conv_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const double* rhoi,
+                    const float* const* A, const float* phi, int acc, hipStream_t s) {
+  const Geom& g = c->g;
+  if (dir == 0) {
+    const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
+    const long nb = (long)nseg * g.Ny * g.Nz;
+    hipLaunchKernelGGL((k_sw_conv_x<NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, rhoi,
+                       A[0], A[1], A[2], phi, c->dparams, g, acc);
+  } else {
+    const int nd = dir == 1 ? g.Ny : g.Nz;
+    const int nseg = (nd + kConvSeg - 1) / kConvSeg;
+    const int seg = (nd + nseg - 1) / nseg;
+    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg);
+    hipLaunchKernelGGL((k_sw_conv_m<NS>), grid, dim3(64, 1 + (NS + 2) / 3), 0, s, rhs, Q, rhoi,
+                       A[0], A[1], A[2], phi, c->dparams, g, dir, seg, acc);
+  }
+}
+
+template <int NS>
+static void diff_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const float* Yi,
+                    const FMet& M, const float* J, const float* lam, const float* mu,
+                    const float* D, hipStream_t s) {
+  const Geom& g = c->g;
+  if (dir == 0) {
+    const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
+    const long nb = (long)nseg * g.Ny * g.Nz;
+    hipLaunchKernelGGL((k_sw_diff_x<0, NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, Yi,
+                       M, J, lam, mu, D, c->dparams, g);
+    hipLaunchKernelGGL((k_sw_diff_x<1, NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, Yi,
+                       M, J, lam, mu, D, c->dparams, g);
+  } else {
+    const int nd = dir == 1 ? g.Ny : g.Nz;
+    const int nseg = (nd + kDiffSeg - 1) / kDiffSeg;
+    const int seg = (nd + nseg - 1) / nseg;
+    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg);
+    hipLaunchKernelGGL(k_sw_visc_m, grid, dim3(64), 0, s, rhs, Q, M, J, lam, mu, g, dir, seg);
+    hipLaunchKernelGGL((k_sw_spec_m<NS>), grid, dim3(64), 0, s, rhs, Q, Yi, M, J, D, c->dparams,
+                       g, dir, seg);
+  }
+}
+
+template <int NS>
+static void stage_end_ns(f3d_ctx* c, int KRK, double* U, const double* Un, double* rhoi,
+                         const double* rhon, double* Q, const float* rhs, double dt,
+                         const double* J, hipStream_t s) {
+  const long n = (long)c->g.Nx * c->g.Ny * c->g.Nz;
+  const int lin = KRK > 1;
+  const double la = KRK == 2 ? 0.25 : 2.0 / 3.0, lb = KRK == 2 ? 0.75 : 1.0 / 3.0;
+  hipLaunchKernelGGL((k_sw_stage_end<NS>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Un, rhoi,
+                     rhon, Q, rhs, dt, J, c->dparams, c->g, lin, la, lb);
+}
+
+int launch_sweep_conv(f3d_ctx* c, int dir, float* rhs, const double* Q, const double* rhoi,
+                      const void* phi, int acc, hipStream_t s) {
+  const float* A[3] = {c->mf[3 * dir], c->mf[3 * dir + 1], c->mf[3 * dir + 2]};
+  switch (c->g.Ns) {
+    case 1: conv_ns<1>(c, dir, rhs, Q, rhoi, A, (const float*)phi, acc, s); break;
+    case 5: conv_ns<5>(c, dir, rhs, Q, rhoi, A, (const float*)phi, acc, s); break;
+    case 9: conv_ns<9>(c, dir, rhs, Q, rhoi, A, (const float*)phi, acc, s); break;
+    default: set_error("sweeps: Ns must be 1, 5 or 9"); return F3D_ERR_UNSUPPORTED;
+  }
+  return hip_check(hipGetLastError(), "convective sweep");
+}
+
+int launch_sweep_diff(f3d_ctx* c, int dir, float* rhs, const double* Q, const void* Yi,
+                      const void* lam, const void* mu, const void* D, hipStream_t s) {
+  FMet M;
+  for (int q = 0; q < 9; ++q) M.m[q] = c->mf[q];
+  const float* J = c->mf[9];
+  const float *y = (const float*)Yi, *l = (const float*)lam, *m = (const float*)mu,
+              *d = (const float*)D;
+  switch (c->g.Ns) {
+    case 1: diff_ns<1>(c, dir, rhs, Q, y, M, J, l, m, d, s); break;
+    case 5: diff_ns<5>(c, dir, rhs, Q, y, M, J, l, m, d, s); break;
+    case 9: diff_ns<9>(c, dir, rhs, Q, y, M, J, l, m, d, s); break;
+    default: set_error("sweeps: Ns must be 1, 5 or 9"); return F3D_ERR_UNSUPPORTED;
+  }
+  return hip_check(hipGetLastError(), "diffusive sweep");
+}
+
+int launch_sweeps(f3d_ctx* c, float* rhs, const double* Q, const double* rhoi, const void* phi,
+                  const void* Yi, const void* lam, const void* mu, const void* D,
+                  hipStream_t s) {
+  for (int d = 0; d < 3; ++d) {
+    int rc = launch_sweep_conv(c, d, rhs, Q, rhoi, phi, d > 0, s);
+    if (rc) return rc;
+  }
+  for (int d = 0; d < 3; ++d) {
+    int rc = launch_sweep_diff(c, d, rhs, Q, Yi, lam, mu, D, s);
+    if (rc) return rc;
+  }
+  return F3D_OK;
+}
+
+int launch_sweep_stage_end(f3d_ctx* c, int KRK, double* U, const double* Un, double* rhoi,
+                           const double* rhon, double* Q, const float* rhs, double dt,
+                           const double* J, hipStream_t s) {
+  switch (c->g.Ns) {
+    case 1: stage_end_ns<1>(c, KRK, U, Un, rhoi, rhon, Q, rhs, dt, J, s); break;
+    case 5: stage_end_ns<5>(c, KRK, U, Un, rhoi, rhon, Q, rhs, dt, J, s); break;
+    case 9: stage_end_ns<9>(c, KRK, U, Un, rhoi, rhon, Q, rhs, dt, J, s); break;
+    default: set_error("sweep stage end: Ns must be 1, 5 or 9"); return F3D_ERR_UNSUPPORTED;
+  }
+  return hip_check(hipGetLastError(), "sweep stage end");
+}
+
+}  // namespace f3d

@@ -97,12 +97,20 @@ typedef struct {
   int32_t recon_kind;  /* F3D_RECON_REFERENCE: UP7 / WENO5-Z / NND (schemes.jl:86-437);
                           F3D_RECON_WENO7: WENO7-Z (Balsara-Shu indicators, Z weights with
                           tau7 = |b0 + 3 b1 - 3 b2 - b3|) in the middle branch */
-  int32_t reserved[4];
+  int32_t stage_kind;  /* F3D_FP32 contexts: F3D_STAGE_FUSED (default) runs f3d_rk3_stage as
+                          per-direction fused sweeps that accumulate every face-flux
+                          difference into one float right-hand side (split and face fluxes
+                          stay on chip); F3D_STAGE_GRANULAR replays the reference's launch
+                          sequence (solver.jl:22-56) with the kernel-granular entry points.
+                          F3D_FP64 contexts and calls with a tag always take the latter. */
+  int32_t reserved[3];
 } f3d_scheme;
 #define F3D_SENSOR_PRESSURE 0
 #define F3D_SENSOR_DUCROS 1
 #define F3D_RECON_REFERENCE 0
 #define F3D_RECON_WENO7 1
+#define F3D_STAGE_FUSED 0
+#define F3D_STAGE_GRANULAR 1
 
 /* ---- context ------------------------------------------------------------ */
 /* Copies the parameter bundles to the device.  Replaces the Adapt-ed structs
@@ -278,6 +286,13 @@ int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int
                 double rtol, double ttol, int8_t* status, int64_t* n_fallback, const int64_t* tag,
                 hipStream_t s);
 
+/* Timing of f3d_nn_chem's parts with HIP events recorded on the call's stream
+ * (no reference counterpart: a measurement hook).  enable = 1 records events
+ * in later calls, 0 stops; ms (host, may be NULL) receives the last timed
+ * call's [network kernel, tile scan + compaction, read-back + Newton
+ * fallback] in milliseconds (synchronizes on that call's end). */
+int f3d_nn_timing(f3d_ctx* ctx, int32_t enable, double* ms);
+
 /* ---- stage level -------------------------------------------------------- */
 /* Registers the block's metrics (9 device pointers, order as above) and J;
  * the context keeps the pointers and, for F3D_FP32, float copies of the
@@ -307,6 +322,26 @@ int f3d_mesh_metrics(f3d_ctx* ctx, const double* x, const double* y, const doubl
  * context-owned device memory (allocated on first use). */
 int f3d_rk3_stage(f3d_ctx* ctx, int32_t KRK, double* U, double* Un, double* rhoi, double* rhon,
                   double* Q, double dt, const int64_t* tag, hipStream_t s);
+/* Pieces of the fused float stage (F3D_FP32 contexts with F3D_STAGE_FUSED,
+ * after f3d_set_metrics), for callers that drive or time them one by one;
+ * f3d_rk3_stage calls them in this order after f3d_mixture and
+ * f3d_shock_sensor.  Together they replace the launches of flowAdvance and
+ * specAdvance (solver.jl:22-56) and the divergence, linComb and c2Prim of
+ * the stage body (solver.jl:304-317).  rhs [5 + Ns][Nx*Ny*Nz] float
+ * (variables rho, rho u, rho v, rho w, E, rho_1.. ; cell a + Nx (b + Ny c)).
+ * f3d_sweep_conv: split -> hybrid reconstruction -> face-flux difference of
+ *   direction dir for all 5 + Ns variables; accumulate = 0 stores, 1 adds.
+ * f3d_sweep_diff: viscous stresses, heat flux, species diffusion and its
+ *   enthalpy flux of direction dir, added (Yi, lambda, mu, D from f3d_mixture).
+ * f3d_sweep_stage_end: U += J dt rhs[0..4], rho_k += J dt rhs[5..], linComb
+ *   with Un, rhon for KRK = 2, 3, then c2Prim (warm-started e -> T). */
+int f3d_sweep_conv(f3d_ctx* ctx, int32_t dir, float* rhs, const double* Q, const double* rhoi,
+                   const float* phi, int32_t accumulate, hipStream_t s);
+int f3d_sweep_diff(f3d_ctx* ctx, int32_t dir, float* rhs, const double* Q, const float* Yi,
+                   const float* lambda, const float* mu, const float* D, hipStream_t s);
+int f3d_sweep_stage_end(f3d_ctx* ctx, int32_t KRK, double* U, const double* Un, double* rhoi,
+                        const double* rhon, double* Q, const float* rhs, double dt,
+                        hipStream_t s);
 /* Chemistry of f3d_time_step's Strang half-steps. */
 #define F3D_CHEM_NONE 0
 #define F3D_CHEM_STIFF 1     /* f3d_chem_stiff, sub_step substeps (eval_gpu_stiff) */
