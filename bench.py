@@ -33,6 +33,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector / matrix), vendor figure
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, vendor figure
+MLP_FLOP_PER_CELL = 2 * (11 * 64 + 64 * 64 + 64 * 64 + 64 * 9)  # 11-64-64-64-9 forward
+OPCOUNT_FILE = os.path.join(ROOT, "profiles", "chem_opcount.json")
 
 
 def _package():
@@ -60,6 +64,7 @@ def log(*a):
 # ---------------------------------------------------------------------------
 def kernel_bytes(name, R, Ns, npts, nf, nc):
     D = 8
+    F = 4
     M = R
     table = {
         "flux_split": npts * (6 * D + 3 * M + 10 * R),       # Q(6) + 3 metrics -> Fp, Fm (5 each)
@@ -73,6 +78,17 @@ def kernel_bytes(name, R, Ns, npts, nf, nc):
         "div": nc * (27 * R + 11 * D),                         # 3x(5+4) faces, J, U RMW
         "div_specs": nc * (6 * Ns * R + (1 + 2 * Ns) * D),
         "c2prim": nc * ((10 + 2 * Ns + 2) * D),                # U, rho_k -> Q(7), rho_k
+        # fused float stage (unit: interior cell); the convective sweep reads
+        # Q (rho, u, v, w, p, e), rho_k, the sweep's 3 metrics and phi and
+        # stores (x) or updates (y, z: read + write) the 5 + Ns right-hand sides
+        "sweep_conv": nc * ((6 + Ns) * D + 4 * F + (5 + Ns) * F * (1 + 2 / 3)),
+        # viscous: u, v, w, T, 9 metrics + J, mu, lambda, rhs 1..4 updated;
+        # species: rho, T, Y_k, D_k, 9 metrics + J, rhs 4 and 5.. updated
+        "sweep_diff": nc * (4 * D + 12 * F + 8 * F + 2 * D + 2 * Ns * F + 10 * F
+                            + 2 * (1 + Ns) * F),
+        # rhs, J, U and rho_k read and written, Un and rho_n read (stages 2, 3:
+        # 2/3 of the stages), Q written and T read
+        "stage_end": nc * ((5 + Ns) * F + D + 2 * (5 + Ns) * D + (5 + Ns) * D * 2 / 3 + 8 * D),
     }
     return table[name]
 
@@ -145,6 +161,12 @@ def kernel_symbols(name, Ns):
                          (f"void f3d::k_spec_viscous_march<{Ns}, float>", 2 / 3)],
         "shock_sensor": [("void f3d::k_shock_sensor<float>", 1.0)],
         "mixture": [(f"void f3d::k_mixture<{Ns}, float>", 1.0)],
+        "sweep_conv": [(f"void f3d::k_sw_conv_x<{Ns}>", 1 / 3),
+                       (f"void f3d::k_sw_conv_m<{Ns}>", 2 / 3)],
+        "sweep_diff": [(f"void f3d::k_sw_diff_x<0, {Ns}>", 1 / 3),
+                       (f"void f3d::k_sw_diff_x<1, {Ns}>", 1 / 3),
+                       ("void f3d::k_sw_visc_m", 2 / 3), (f"void f3d::k_sw_spec_m<{Ns}>", 2 / 3)],
+        "stage_end": [(f"void f3d::k_sw_stage_end<{Ns}>", 1.0)],
     }.get(name)
 
 
@@ -205,6 +227,52 @@ def time_kernels(ctx, torch, Q, rhoi, metrics, J, reps=10):
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1) / (reps * len(fns)))
         out[name] = (best, len(fns))
+    return out
+
+
+def _best_of(torch, fns, reps):
+    """Mean launch duration of `fns` (HIP events on the launch stream), the
+    fastest of five timed batches of `reps` rounds (a host stall while the
+    launches are enqueued would otherwise count as GPU time)."""
+    for fn in fns:
+        fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            for fn in fns:
+                fn()
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) / (reps * len(fns)))
+    return best
+
+
+def time_fused_kernels(ctx, torch, U, Un, Q, rhoi, rhon, dt, reps=10):
+    """Mean launch duration of each kernel of the fused float stage on the
+    state a run left behind: {name: (ms, launches per stage)}.  The diffusive
+    sweep of a direction is its viscous and species kernels together; the
+    stage end runs on copies of the state (KRK = 2, with linComb)."""
+    w = ctx.workspace
+    rhs = ctx.rhs_zeros()
+    out = {
+        "mixture": (_best_of(torch, [lambda: ctx.mixture(Q, rhoi, w("Yi"), w("lambda"), w("mu"),
+                                                          w("D"))], reps), 1),
+        "shock_sensor": (_best_of(torch, [lambda: ctx.shock_sensor(w("phi"), Q)], reps), 1),
+        "sweep_conv": (_best_of(torch, [
+            (lambda d=d: ctx.sweep_conv(d, rhs, Q, rhoi, w("phi"), d > 0)) for d in range(3)],
+            reps), 3),
+        "sweep_diff": (_best_of(torch, [
+            (lambda d=d: ctx.sweep_diff(d, rhs, Q, w("Yi"), w("lambda"), w("mu"), w("D")))
+            for d in range(3)], reps), 3),
+    }
+    U2, Q2, r2 = U.clone(), Q.clone(), rhoi.clone()
+    out["stage_end"] = (_best_of(torch, [lambda: ctx.sweep_stage_end(2, U2, Un, r2, rhon, Q2, rhs,
+                                                                     dt)], reps), 1)
+    del U2, Q2, r2, rhs
     return out
 
 
@@ -296,10 +364,45 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1)
     cms = [a.elapsed_time(b) for a, b in chem_ms] or [0.0]
-    kt = time_kernels(ctx, torch, Q, rhoi, case.grid.metrics, case.grid.J, kernel_reps)
+    fused = prec == "fp32" and (scheme or {}).get("stage_kind", abi.STAGE_FUSED) == abi.STAGE_FUSED
+    if fused:
+        kt = time_fused_kernels(ctx, torch, U, Un, Q, rhoi, rhon, case.dt, kernel_reps)
+    else:
+        kt = time_kernels(ctx, torch, Q, rhoi, case.grid.metrics, case.grid.J, kernel_reps)
+    extra = {}
+    if chemistry and surrogate is not None:
+        # the surrogate step's parts (network kernel, scan + compaction,
+        # read-back + Newton fallback), event-timed inside the library on the
+        # launch stream, over two further calls from copies of the end state
+        Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
+        ctx.nn_timing(True)
+        parts, nfb = [], []
+        for _ in range(2):
+            nfb.append(ctx.nn_chem(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6))
+            parts.append(ctx.nn_timing())
+        ctx.nn_timing(False)
+        extra["nn_parts_ms"] = [sum(p[k] for p in parts) / len(parts) for k in range(3)]
+        extra["nn_parts_fallback"] = sum(nfb) / len(nfb)
+        del Uc, Qc, rc
+    elif chemistry:
+        # one more Newton call from copies of the end state with the
+        # iteration counts returned (the FP64 work of the launch)
+        Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
+        iters = torch.zeros((N[2], N[1], N[0]), dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a0.record()
+        ctx.chem_newton(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6, iters=iters)
+        a1.record()
+        torch.cuda.synchronize()
+        it = iters.abs().to(torch.int64)
+        extra["newton_ms"] = a0.elapsed_time(a1)
+        extra["newton_iterations"] = int(it.sum())
+        extra["newton_cells"] = int((it > 0).sum())
+        del Uc, Qc, rc, iters, it
     return ctx, dict(ms_total=ms, wall_s=wall, Q=Q, rhoi=rhoi, U=U, kernels=kt,
                      chem_ms_mean=sum(cms) / len(cms), chem_calls=len(cms),
-                     fallback=fallback)
+                     fallback=fallback, **extra)
 
 
 def field_errors(torch, a, b, names):
@@ -452,6 +555,32 @@ def cpu_baseline(mech, size, steps, min_seconds=10.0):
                       f"Strang Newton chemistry), {dt:.1f} s"}
 
 
+REF_CHEM = os.path.join(ROOT, "oracle", "_ref", "chemical_test")
+
+
+def cpu_baseline_reference(t_end="4e-3"):
+    """The reference's own native chemistry, NN/H2/chemical_test.cpp compiled
+    from its source (oracle/_ref, by oracle/Makefile): a 0-D H2/O2 reactor at
+    1500 K and 10 atm advanced by its point-implicit step (rates, Jacobian,
+    pivoted solve, GetT) with dt = 1e-8 s to t_end (4e5 steps, ~6 s on one
+    core, its per-step output to /dev/null); reactor-steps per second on one
+    core.  None where the binary is absent."""
+    import subprocess
+    if not os.path.exists(REF_CHEM):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        subprocess.run([REF_CHEM, "1e-8", t_end, "1500", "10"], cwd=td, check=True,
+                       stdout=subprocess.DEVNULL)
+        dt = time.perf_counter() - t0
+    steps = round(float(t_end) / 1e-8)
+    return {"value": steps / dt, "unit": "reactor-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"NN/H2/chemical_test.cpp (compiled from the reference's source), one "
+                      f"0-D H2/O2 reactor, 1500 K / 10 atm, {steps} point-implicit steps of "
+                      f"1e-8 s in {dt:.1f} s, per-step output discarded"}
+
+
 def bench_air(a, torch, abi, cases, mm):
     """The shock-laden mixing layer of 5-species air, non-reacting, hybrid
     WENO5-Z / UP7 switched by the Ducros sensor at 0.1: one RK3 stage (the
@@ -479,8 +608,9 @@ def bench_air(a, torch, abi, cases, mm):
                 "GBps": b / (ms * 1e-3) / 1e9, "frac_hbm": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     print(json.dumps({
         "metric": "cell-updates/s per RK stage, FP32 flux vs FP64 (non-reacting air)",
-        "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": stages,
-        "warmup": a.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "value": nc / (ms32 * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 3 * ms32, "ms_per_stage": ms32,
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 flux, f64 state", "data": "synthetic",
         "config": {"workload": f"shock-laden mixing layer {N[0]}^3, 5-species air, non-reacting, "
                                "WENO5-Z/UP7 switched by the Ducros sensor at 0.1"},
@@ -554,6 +684,8 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=48)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the per-step accuracy decomposition (untimed runs)")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--workload", choices=("reacting", "air", "jet"), default="reacting",
@@ -611,8 +743,10 @@ def main():
     errs = field_errors(torch, r32["Q"], r64["Q"], ["rho", "u", "v", "w", "p", "T", "e"])
     acc = state_errors(r32, r64, mech)
     value = nc / (ms32 * 1e-3)
+    c32.close()
+    c64.close()
 
-    # per-kernel roofline (FP32 path); dominant = largest time per stage
+    # ---- per-kernel rooflines ------------------------------------------
     Ns = mech.Ns
     npts = (N[0] + 8) * (N[1] + 8) * (N[2] + 8)
     nf = (N[0] + 1) * N[1] * N[2]  # faces per sweep (equal for x, y, z on a cube)
@@ -624,17 +758,68 @@ def main():
             ktab.setdefault(prec, {})[name] = {
                 "ms": ms, "launches_per_stage": per, "alg_bytes": b,
                 "GBps": b / (ms * 1e-3) / 1e9, "frac_hbm": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    dom = max(ktab["fp32"], key=lambda k: ktab["fp32"][k]["ms"] * ktab["fp32"][k]["launches_per_stage"])
-    dk = ktab["fp32"][dom]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": dk["GBps"] / HBM_PEAK_GBS,
-                "traffic": traffic_for(kernel_symbols(dom, Ns)),
-                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["ms"]}
+    # chemistry: FP64 work from the oracle's operation counts
+    # (profiles/chem_opcount.json) over the event-timed launches
+    ops = json.load(open(OPCOUNT_FILE))["mechanisms"]["h2_lidryer"]
+    hot = int((case.Q[5][(slice(4, -4),) * 3] > 500.0).sum())
+    nn_ms, nn_scan_ms, nn_fb_ms = r32["nn_parts_ms"]
+    nn_f64 = hot * ops["surrogate_epilogue"]["flop"]
+    nn_bf16 = hot * MLP_FLOP_PER_CELL
+    chem_tab = {
+        "k_nn_chem": {"ms": nn_ms, "launches_per_step": 2, "cells": hot,
+                      "fp64_flop": nn_f64, "fp64_TFLOPs": nn_f64 / (nn_ms * 1e-3) / 1e12,
+                      "bf16_mfma_flop": nn_bf16,
+                      "bf16_mfma_TFLOPs": nn_bf16 / (nn_ms * 1e-3) / 1e12,
+                      "bf16_mfma_frac": nn_bf16 / (nn_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS},
+        "nn_scan_compact": {"ms": nn_scan_ms, "launches_per_step": 2},
+        "newton_fallback": {"ms": nn_fb_ms, "launches_per_step": 2,
+                            "cells": r32["nn_parts_fallback"]},
+        "k_chem_newton_fp64_plain": {
+            "ms": r64["newton_ms"], "launches_per_step": 2, "cells": r64["newton_cells"],
+            "iterations": r64["newton_iterations"],
+            "fp64_flop": r64["newton_iterations"] * ops["newton_iteration"]["flop"]
+            + r64["newton_cells"] * ops["newton_cell_overhead"]["flop"]},
+    }
+    kn = chem_tab["k_chem_newton_fp64_plain"]
+    kn["fp64_TFLOPs"] = kn["fp64_flop"] / (kn["ms"] * 1e-3) / 1e12
+    kn["fp64_frac"] = kn["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
+    chem_tab["k_nn_chem"]["fp64_frac"] = chem_tab["k_nn_chem"]["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
+    # the dominant kernel of the FP32 + surrogate step: largest time per step
+    share = {k: v["ms"] * v["launches_per_stage"] * 3 for k, v in ktab["fp32"].items()}
+    share["k_nn_chem"] = nn_ms * 2
+    share["newton_fallback"] = nn_fb_ms * 2
+    dom = max(share, key=share.get)
+    if dom == "k_nn_chem":
+        kc = chem_tab["k_nn_chem"]
+        roofline = {"bound": "mfma", "kernel": "k_nn_chem (FP64 gate epilogue bound)",
+                    "achieved": kc["fp64_TFLOPs"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": kc["fp64_frac"], "traffic": None,
+                    "alg_fp64_flop_per_launch": nn_f64, "avg_launch_ms": nn_ms,
+                    "bf16_mfma_frac": kc["bf16_mfma_frac"]}
+    elif dom == "newton_fallback":
+        roofline = {"bound": "mfma", "kernel": "k_chem Newton fallback", "achieved": None,
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
+                    "traffic": None, "avg_launch_ms": nn_fb_ms}
+    else:
+        dk = ktab["fp32"][dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": dk["GBps"] / HBM_PEAK_GBS,
+                    "traffic": traffic_for(kernel_symbols(dom, Ns)),
+                    "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_ms": dk["ms"]}
+    roofline["ms_per_step"] = share[dom]
+    roofline["timing"] = ("HIP events on the launch stream; stage kernels: fastest of five "
+                          "batches of launches on the run's end state; chemistry parts: mean of "
+                          "two calls on copies of it")
 
-    cpu = None
+    accuracy = None
+    if not a.no_accuracy:
+        accuracy = accuracy_decomposition(case, mech, a.steps, torch, abi, sur)
+
+    cpu = cpu_ref = None
     if not a.no_cpu:
         log(f"[bench] CPU baseline: oracle {a.cpu_size}^3 x >= {a.cpu_steps} steps")
         cpu = cpu_baseline(mech, a.cpu_size, a.cpu_steps)
+        cpu_ref = cpu_baseline_reference()
 
     nfb = r32["fallback"]
     out = {
@@ -648,22 +833,24 @@ def main():
                                "UP7/WENO5-Z/NND, viscous + diffusion, Strang chemistry",
                    "cells": nc, "step": "one time step = chemistry dt/2 + 3 RK3 stages + "
                    "chemistry dt/2; value = cells x 3 / step time",
-                   "dt_s": case.dt, "precision": "FP32 flux, FP64 state, BF16 MLP chemistry "
-                   "with FP64 Newton fallback"},
+                   "dt_s": case.dt, "precision": "FP32 flux (fused sweeps), FP64 state, BF16 "
+                   "MLP chemistry with FP64 Newton fallback"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_reference": cpu_ref,
         "fp64_plain": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
-                       "chem_ms_per_call": r64["chem_ms_mean"]},
+                       "chem_ms_per_call": r64["chem_ms_mean"],
+                       "stage": "FP64 kernel-granular (the reference's launch sequence)"},
         "fp32_nn": {"ms_per_stage": ms32, "chem_ms_per_call": r32["chem_ms_mean"],
                     "fallback_fraction": sum(nfb) / (nc * len(nfb))},
         "speedup_vs_fp64_plain": ms64 / ms32,
         "error_vs_fp64_plain": acc,
+        "accuracy": accuracy,
         "error_fields_vs_fp64_plain": errs,
         "kernels": ktab,
+        "chemistry_kernels": chem_tab,
     }
     print(json.dumps(out))
-    c32.close()
-    c64.close()
 
 
 if __name__ == "__main__":
