@@ -267,6 +267,50 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
   return t1;
 }
 
+// GetT for an energy inside the small jump of the NASA-7 fits at their
+// breakpoint, where e(T) has no root.  get_T's update then alternates
+// between the roots of the two fits' continuations across the breakpoint (a
+// 2-cycle, reached within a few updates) until its 30 updates run out, and
+// returns the cycle point the parity of the remaining updates selects.  This
+// is get_T's iteration (same start, update, clamp and stopping tests) that,
+// once x_m repeats x_{m-2} to 1e-9 K, returns the point x_30 would be
+// (x_m for an even number of remaining updates, x_{m-1} for an odd one)
+// instead of doing the remaining updates.  Off the cycle it is get_T.
+template <int NS>
+__device__ inline double get_T_gap(const DevParams* P, double ein, const double* rhoi) {
+  const double tol = 1e-3;
+  const double tmin = P->min_temp, tmax = P->max_temp;
+  const double emin = internal_energy<NS>(P, tmin, rhoi);
+  const double emax = internal_energy<NS>(P, tmax, rhoi);
+  if (ein < emin) return tmin - (emin - ein) / cv_mix<NS>(P, tmin, rhoi);
+  if (ein > emax) return tmax - (emax - ein) / cv_mix<NS>(P, tmax, rhoi);
+  double As = 0;
+#pragma unroll
+  for (int n = 0; n < NS; ++n) As += (P->lo[n][0] - 1) * P->Ru / P->mw[n] * rhoi[n];
+  double t1 = ein / As;
+  if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
+  double x1 = 0.0, x2 = 0.0;  // t1 one and two updates ago
+  for (int m = 0; m < 30; ++m) {
+    const double e1 = internal_energy<NS>(P, t1, rhoi);
+    const double cv = cv_mix<NS>(P, t1, rhoi);
+    double dt = (ein - e1) / cv;
+    if (dt > 100.0) {
+      dt = 100.0;
+    } else if (dt < -100.0) {
+      dt = -100.0;
+    } else if (fabs(dt) < tol) {
+      break;
+    } else if (t1 + dt == t1) {
+      break;
+    }
+    if (m >= 2 && fabs(t1 - x2) <= 1e-9) return ((30 - m) % 2 == 0) ? t1 : x1;
+    x2 = x1;
+    x1 = t1;
+    t1 += dt;
+  }
+  return t1;
+}
+
 // GetT warm-started from a nearby earlier temperature (the cell's T before
 // a stage or a chemistry step): the same Newton update, 1e-3 K tolerance,
 // +-100 K clamp and 30-update limit as get_T, from T_guess instead of the
@@ -275,10 +319,10 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
 // the 30 updates, or a result outside the range
 // (the reference's linear extrapolation region), takes get_T itself.  Both
 // stop within the tolerance of the same root, so they differ by < 1e-3 K.
-// Unlike get_T it also stops when the update flips sign without shrinking:
-// an energy inside the small jump of the NASA-7 fits at their breakpoint
-// (e.g. a stream at 1000 K) has no exact root, and get_T then spends its 30
-// updates oscillating across the breakpoint.
+// An update that flips sign without shrinking marks an energy inside the
+// small jump of the NASA-7 fits at their breakpoint (e.g. a stream at
+// 1000 K), which has no exact root: there, and for any result within 0.5 K
+// of a breakpoint, the result is get_T's own end point (get_T_gap).
 template <int NS>
 __device__ inline double get_T_from(const DevParams* P, double ein, const double* rhoi,
                                     double T_guess) {
@@ -303,15 +347,20 @@ __device__ inline double get_T_from(const DevParams* P, double ein, const double
       done = true;
       break;
     } else if (dt * dt_prev < 0.0 && fabs(dt) >= 0.5 * fabs(dt_prev)) {
-      // the update flips sign without shrinking: the root sits in the small
-      // energy jump of the NASA-7 fits at their breakpoint (no exact root)
-      done = true;
-      break;
+      // the update flips sign without shrinking: the energy sits in the
+      // jump of the NASA-7 fits at their breakpoint (no exact root)
+      return get_T_gap<NS>(P, ein, rhoi);
     }
     dt_prev = dt;
     t1 += dt;
   }
   if (!done || !(t1 >= tmin && t1 <= tmax)) return get_T<NS>(P, ein, rhoi, nullptr);
+  // within 0.5 K of a breakpoint (the jumps of the fits are <= 0.2 K wide):
+  // where the jump is narrower than the tolerance, or the fits overlap (two
+  // roots), the start decides where the update stops; take get_T's end point
+#pragma unroll
+  for (int n = 0; n < NS; ++n)
+    if (fabs(t1 - P->sep[n]) <= 0.5) return get_T_gap<NS>(P, ein, rhoi);
   return t1;
 }
 

@@ -496,11 +496,52 @@ double or_get_T(const or_dims* d, const or_thermo* th, double ein, const double*
   return t1;
 }
 
+// GetT for an energy inside the jump of the NASA-7 fits at their breakpoint
+// (no root): or_get_T's iteration, which there alternates between the roots
+// of the two fits' continuations until its updates run out; once x_m repeats
+// x_{m-2} to 1e-9 K the point x_maxiter is returned by parity (x_m for an
+// even number of remaining updates, x_{m-1} for an odd one).  The library's
+// get_T_gap; pinned by agreement with or_get_T inside the jump (tests/).
+double or_get_T_gap(const or_dims* d, const or_thermo* th, double ein, const double* rhoi,
+                    double tol, int32_t maxiter, double tmin, double tmax) {
+  Grid g(d);
+  double emin = or_internal_energy(d, th, tmin, rhoi);
+  double emax = or_internal_energy(d, th, tmax, rhoi);
+  if (ein < emin) return tmin - (emin - ein) / or_cv(d, th, tmin, rhoi);
+  if (ein > emax) return tmax - (emax - ein) / or_cv(d, th, tmax, rhoi);
+  double As = 0;
+  for (int n = 0; n < g.Ns; ++n) As += (th->coeffs_lo[7 * n] - 1) * th->Ru / th->mw[n] * rhoi[n];
+  double t1 = ein / As;
+  if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
+  double x1 = 0.0, x2 = 0.0;
+  for (int m = 0; m < maxiter; ++m) {
+    double e1 = or_internal_energy(d, th, t1, rhoi);
+    double cv = or_cv(d, th, t1, rhoi);
+    double dt = (ein - e1) / cv;
+    if (dt > 100.0) {
+      dt = 100.0;
+    } else if (dt < -100.0) {
+      dt = -100.0;
+    } else if (std::fabs(dt) < tol) {
+      break;
+    } else if (t1 + dt == t1) {
+      break;
+    }
+    if (m >= 2 && std::fabs(t1 - x2) <= 1e-9) return ((maxiter - m) % 2 == 0) ? t1 : x1;
+    x2 = x1;
+    x1 = t1;
+    t1 += dt;
+  }
+  return t1;
+}
+
 // GetT warm-started from T_guess (the library's get_T_from, used on the FP32
 // stage's c2Prim and by the surrogate step): the Newton update of GetT from
-// T_guess; a guess or result outside [tmin, tmax] takes or_get_T.  Parity
-// unpinned against the reference (it has no warm start); pinned here by
-// agreement with or_get_T to the 1e-3 K tolerance (tests/).
+// T_guess; a guess or result outside [tmin, tmax] takes or_get_T, an energy
+// inside the NASA-7 breakpoint jump (an update that flips sign without
+// shrinking) or a result within 0.5 K of a breakpoint or_get_T_gap.  Parity unpinned against the reference (it has no
+// warm start); pinned here by agreement with or_get_T to the 1e-3 K
+// tolerance, and inside the jump to 1e-6 K (tests/).
 double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const double* rhoi,
                      double T_guess, double tol, int32_t maxiter, double tmin, double tmax) {
   if (!(T_guess >= tmin && T_guess <= tmax))
@@ -523,15 +564,18 @@ double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const do
       done = true;
       break;
     } else if (dt * dt_prev < 0.0 && std::fabs(dt) >= 0.5 * std::fabs(dt_prev)) {
-      // the update flips sign without shrinking: the root sits in the small
-      // energy jump of the NASA-7 fits at their breakpoint (no exact root)
-      done = true;
-      break;
+      // the update flips sign without shrinking: the energy sits in the
+      // jump of the NASA-7 fits at their breakpoint (no exact root)
+      return or_get_T_gap(d, th, ein, rhoi, tol, maxiter, tmin, tmax);
     }
     dt_prev = dt;
     t1 += dt;
   }
   if (!done || !(t1 >= tmin && t1 <= tmax)) return or_get_T(d, th, ein, rhoi, tol, maxiter, tmin, tmax, nullptr);
+  // within 0.5 K of a breakpoint: the reference's own end point
+  for (int n = 0; n < d->Ns; ++n)
+    if (std::fabs(t1 - th->coeffs_sep[n]) <= 0.5)
+      return or_get_T_gap(d, th, ein, rhoi, tol, maxiter, tmin, tmax);
   return t1;
 }
 

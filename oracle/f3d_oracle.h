@@ -49,6 +49,8 @@ double or_internal_energy(const or_dims* d, const or_thermo* th, double T, const
 double or_cv(const or_dims* d, const or_thermo* th, double T, const double* rhoi);
 double or_get_T(const or_dims* d, const or_thermo* th, double e, const double* rhoi,
                 double tol, int32_t maxiter, double tmin, double tmax, int32_t* iters);
+double or_get_T_gap(const or_dims* d, const or_thermo* th, double e, const double* rhoi,
+                    double tol, int32_t maxiter, double tmin, double tmax);
 double or_get_T_from(const or_dims* d, const or_thermo* th, double e, const double* rhoi,
                      double T_guess, double tol, int32_t maxiter, double tmin, double tmax);
 void or_h_specs(const or_dims* d, const or_thermo* th, double T, double* h);

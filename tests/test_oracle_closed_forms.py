@@ -372,16 +372,27 @@ def test_getT_warm_start_agrees_with_getT(h2):
     assert orc.get_T_from(e, r, 9000.0) == orc.get_T(e, r)[0]
     # an energy inside the NASA-7 jump at 1000 K (H2-rich: ~0.2 K wide) has no
     # root; the reference start spends its 30 updates oscillating across the
-    # breakpoint, the warm start stops at the first non-shrinking flip, both
-    # inside the jump
-    r = np.zeros(h2.Ns)
-    r[h2.index("H2")], r[h2.index("N2")] = 0.3, 0.01
-    elo, ehi = orc.internal_energy(1000.0 - 1e-9, r), orc.internal_energy(1000.0, r)
-    gap = (ehi - elo) / orc.cv(1000.0, r)
-    assert gap > 1e-2
-    T_ref, it = orc.get_T(0.5 * (elo + ehi), r)
-    assert it == 30 and abs(T_ref - 1000.0) <= gap
-    assert abs(orc.get_T_from(0.5 * (elo + ehi), r, 1000.3) - 1000.0) <= gap
+    # breakpoint and ends on the cycle point the parity selects.  The warm
+    # start detects the non-shrinking flip and returns that same point
+    # (get_T_gap: the reference iteration cut short once the cycle repeats),
+    # whatever the guess and wherever in the jump the energy lies
+    for comp in ((0.3, 0.01), (0.02, 0.9), (0.1, 0.2)):
+        r = np.zeros(h2.Ns)
+        r[h2.index("H2")], r[h2.index("N2")] = comp
+        r[h2.index("O2")] = 0.05
+        elo, ehi = orc.internal_energy(1000.0 - 1e-9, r), orc.internal_energy(1000.0, r)
+        gap = (ehi - elo) / orc.cv(1000.0, r)
+        assert abs(gap) > 1e-3, gap
+        for frac in (0.05, 0.3, 0.5, 0.77, 0.95):
+            e = elo + frac * (ehi - elo)
+            T_ref, it = orc.get_T(e, r)
+            assert it == 30 and abs(T_ref - 1000.0) <= abs(gap) + 1e-9
+            for guess in (999.7, 1000.0, 1000.3, 1003.0, 960.0):
+                Tw = orc.get_T_from(e, r, guess)
+                assert abs(Tw - T_ref) < 1e-6, (comp, frac, guess, Tw, T_ref)
+            # maxiter parity: an odd update budget ends on the other side
+            T29, it29 = orc.get_T(e, r, maxiter=29)
+            assert it29 == 29 and abs(T29 - T_ref) > 0.5 * abs(gap)
 
 
 def test_gibbs_equals_h_minus_s(h2):
