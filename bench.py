@@ -434,7 +434,39 @@ def state_errors(ra, rb, mech):
 ACC_BAR = {"T_K": 1.0, "Y": 1e-3}  # the accuracy bar of the FP32 + surrogate path
 
 
-def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6)):
+def branch_codes(torch, phi, phi1, phi2, NG=4):
+    """Hybrid-switch branch (0 upwind, 1 WENO, 2 NND; schemes.jl:96-100) of
+    every interior face of the x, y, z sweeps from one stage's sensor phi
+    ((Nz+8, Ny+8, Nx+8)), face i-1/2 at max(phi[i-1], phi[i]), compared in
+    phi's own precision as the kernels compare."""
+    Nzt, Nyt, Nxt = phi.shape
+    Nx, Ny, Nz = Nxt - 2 * NG, Nyt - 2 * NG, Nzt - 2 * NG
+    p = phi
+    a1 = torch.tensor(phi1, dtype=p.dtype, device=p.device)
+    a2 = torch.tensor(phi2, dtype=p.dtype, device=p.device)
+    fx = torch.maximum(p[NG:NG + Nz, NG:NG + Ny, NG - 1:NG + Nx], p[NG:NG + Nz, NG:NG + Ny, NG:NG + Nx + 1])
+    fy = torch.maximum(p[NG:NG + Nz, NG - 1:NG + Ny, NG:NG + Nx], p[NG:NG + Nz, NG:NG + Ny + 1, NG:NG + Nx])
+    fz = torch.maximum(p[NG - 1:NG + Nz, NG:NG + Ny, NG:NG + Nx], p[NG:NG + Nz + 1, NG:NG + Ny, NG:NG + Nx])
+    return [(f >= a1).to(torch.int8) + (f >= a2).to(torch.int8) for f in (fx, fy, fz)]
+
+
+def mark_flips(torch, m, ca, cb):
+    """OR into the cell mask m (Nz, Ny, Nx) the two cells of every face whose
+    branch differs between codes ca and cb; returns the number of such faces."""
+    n = 0
+    for d, (x, y) in enumerate(zip(ca, cb)):
+        f = x != y
+        n += int(f.sum())
+        if d == 0:
+            m |= f[:, :, :-1] | f[:, :, 1:]
+        elif d == 1:
+            m |= f[:, :-1, :] | f[:, 1:, :]
+        else:
+            m |= f[:-1] | f[1:]
+    return n
+
+
+def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), radius=2):
     """Untimed runs of `steps` time steps from the same initial state, each
     compared after every step with FP64 plain (FP64 stage + FP64 Newton
     chemistry) by the interior's max |dT| and max_k |dY_k|:
@@ -444,8 +476,14 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6)):
       fp64_perturbed  FP64 plain from velocities perturbed by 1e-7 relative
                     (float rounding of the initial state): the flow's own
                     sensitivity to a perturbation of that size.
+    The hybrid scheme's switch is discontinuous in the sensor, so a
+    perturbation of any size flips the branch at the faces whose sensor sits
+    at a threshold; every stage's branch codes are compared with the FP64
+    plain run's, and T is also reported away from the cells within `radius`
+    of a face flipped at any stage so far (T_linf_away_K).
     Returns per-step curves, the values at `marks` and at the last step, and
-    the first step at which each path leaves the bar."""
+    the first step at which each path leaves the bar (in Linf, and away from
+    the flips)."""
     N = case.grid.Nx, case.grid.Ny, case.grid.Nz
     sl = (slice(4, -4),) * 3
     keep_all = steps <= 40
@@ -465,19 +503,33 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6)):
         Un, rn = ctx.zeros(5), ctx.zeros(mech.Ns)
         if nn:
             sur.attach(ctx)
-        chem = abi.CHEM_SURROGATE if nn else abi.CHEM_NEWTON
+        phi1, phi2 = ctx.scheme["hybrid_phi1"], ctx.scheme["hybrid_phi2"]
+
+        def half():
+            if nn:
+                ctx.nn_chem(U, Q, rhoi, case.dt / 2, 50, 1e-10, 1e-6)
+            else:
+                ctx.chem_newton(U, Q, rhoi, case.dt / 2, 50, 1e-10, 1e-6)
+            ctx.fill(Q, U, rhoi)
+        # f3d_time_step's sequence, with every stage's branch codes
         for s in range(1, steps + 1):
-            ctx.time_step(U, Un, rhoi, rn, Q, case.dt, chem, 1, 50, 1e-10, 1e-6)
-            sink(s, Q, rhoi)
+            half()
+            codes = []
+            for k in (1, 2, 3):
+                ctx.rk3_stage(k, U, Un, rhoi, rn, Q, case.dt)
+                codes.append(branch_codes(torch, ctx.workspace("phi"), phi1, phi2))
+            half()
+            sink(s, Q, rhoi, codes)
         torch.cuda.synchronize()
         ctx.close()
 
     def TY(Q, rhoi):
         return Q[5][sl].clone(), (rhoi[(slice(None),) + sl] / Q[0][sl].unsqueeze(0)).float()
 
-    ref = {}
+    ref, ref_codes = {}, {}
 
-    def keep(s, Q, rhoi):
+    def keep(s, Q, rhoi, codes):
+        ref_codes[s] = codes
         if keep_all or s in marks or s == steps:
             ref[s] = TY(Q, rhoi)
     log(f"[bench] accuracy: FP64 plain reference, {steps} steps")
@@ -486,30 +538,48 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6)):
     for name, prec, nn, pert in (("fp32_newton", "fp32", False, 0.0), ("fp64_nn", "fp64", True, 0.0),
                                  ("fp32_nn", "fp32", True, 0.0),
                                  ("fp64_perturbed", "fp64", False, 1e-7)):
-        curve = {"step": [], "T_linf_K": [], "Y_linf": [], "cells_T_over_1K": []}
+        curve = {"step": [], "T_linf_K": [], "Y_linf": [], "cells_T_over_1K": [],
+                 "face_flips": [], "cells_near_flips": [], "T_linf_away_K": []}
+        near = torch.zeros(N[::-1], dtype=torch.bool, device="cuda")
+        nflips = [0]
 
-        def cmp(s, Q, rhoi):
+        def cmp(s, Q, rhoi, codes):
+            for ca, cb in zip(ref_codes[s], codes):
+                nflips[0] += mark_flips(torch, near, ca, cb)
             if s not in ref:
                 return
             T, Y = TY(Q, rhoi)
             Tr, Yr = ref[s]
             dT = (T - Tr).abs()
+            k = 2 * radius + 1
+            m = torch.nn.functional.max_pool3d(near[None, None].float(), k, stride=1,
+                                               padding=radius)[0, 0] > 0
             curve["step"].append(s)
             curve["T_linf_K"].append(float(dT.max()))
             curve["Y_linf"].append(float((Y - Yr).abs().max()))
             curve["cells_T_over_1K"].append(int((dT > 1.0).sum()))
+            curve["face_flips"].append(nflips[0])
+            curve["cells_near_flips"].append(int(m.sum()))
+            curve["T_linf_away_K"].append(float(dT[~m].max()))
         log(f"[bench] accuracy: {name}")
         run(prec, nn, cmp, pert)
         first = next((s for s, t, y in zip(curve["step"], curve["T_linf_K"], curve["Y_linf"])
                       if t > ACC_BAR["T_K"] or y > ACC_BAR["Y"]), None)
+        first_away = next((s for s, t, y in zip(curve["step"], curve["T_linf_away_K"],
+                                                curve["Y_linf"])
+                           if t > ACC_BAR["T_K"] or y > ACC_BAR["Y"]), None)
         at = {str(s): {"T_linf_K": curve["T_linf_K"][i], "Y_linf": curve["Y_linf"][i],
-                       "cells_T_over_1K": curve["cells_T_over_1K"][i]}
+                       "cells_T_over_1K": curve["cells_T_over_1K"][i],
+                       "T_linf_away_from_flips_K": curve["T_linf_away_K"][i],
+                       "face_flips": curve["face_flips"][i]}
               for i, s in enumerate(curve["step"]) if s in marks or s == steps}
-        out[name] = {"at_step": at, "first_step_over_bar": first, "curve": curve}
-    del ref
+        out[name] = {"at_step": at, "first_step_over_bar": first,
+                     "first_step_over_bar_away_from_flips": first_away, "curve": curve}
+        del near
+    del ref, ref_codes
     torch.cuda.empty_cache()
     return {"bar": ACC_BAR, "steps": steps, "vs": "FP64 plain (FP64 stage + FP64 Newton chemistry)",
-            **out}
+            "flip_radius_cells": radius, **out}
 
 
 def cpu_baseline(mech, size, steps, min_seconds=10.0):

@@ -188,6 +188,7 @@ class Context:
                  stage_kind=STAGE_FUSED)
         if scheme:
             s.update(scheme)
+        self.scheme = s
         m = mech
         keep = [_f64(x) for x in (m.mw, m.coeffs_sep, m.coeffs_lo, m.coeffs_hi, m.visc_poly,
                                   m.conduct_poly, m.binarydiff_poly)]

@@ -256,7 +256,7 @@ LABEL_SCALE = 1e-6  # asinh label scale: dY below it is linear, above it logarit
 LABEL_REL = 0.1     # label_mode 3: below Y = LABEL_SCALE / LABEL_REL the scale follows 0.1 Y
 LABEL_FLOOR = 1e-14  # ... down to this floor (species at zero)
 Y_FLOOR = 1e-12     # inputs see Y below this as this (traces below it do not move the features)
-CHAIN_MAX = 0.5     # chain-explosiveness gate: largest real eigenvalue of dt d wdot / d c
+CHAIN_MAX = 0.05    # chain-explosiveness gate: largest real eigenvalue of dt d wdot / d c
 CHAIN_SPECIES = ("H", "O", "OH")  # ... over the chain carriers
 
 

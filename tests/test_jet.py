@@ -92,44 +92,78 @@ def test_jet_time_step_parity():
 @pytest.mark.gpu
 def test_jet_time_step_with_air_surrogate():
     """Three time steps of the jet at the fixed dt of 1.2e-8 s the air
-    surrogate was trained for (chemistry half-steps of 6e-9 s): FP32 stage +
-    BF16 MLP air chemistry with gates and Newton fallback (f3d_time_step,
-    surrogate form) against FP64 stage + FP64 Newton chemistry.  Bars:
-    species mass fractions within 1e-3 everywhere; temperature mean
-    |difference| within 0.02 K and within 1 K in all but 0.1 % of the cells.  The exception is
-    the hybrid scheme's branch switch: on this coarse block a shift of 4e-7
-    in Y_O alone, given to the FP64 Newton path, moves one cell at the jet
-    edge by 8.9 K after two steps (the switch picks the other branch there),
-    and the surrogate's species differ from Newton's by that much."""
+    surrogate was trained for (chemistry half-steps of 6e-9 s), against FP64
+    stage + FP64 Newton air chemistry: FP32 stage + BF16 MLP air chemistry
+    with gates and Newton fallback (the benchmarked form) and its two parts,
+    the surrogate with the FP64 stage and the FP32 stage with Newton.  Bars
+    (north star): Y <= 1e-3 everywhere; T <= 1 K everywhere except next to a
+    face where the hybrid scheme took a different branch than in the
+    reference run at some stage (within 2 cells).  The switch is
+    discontinuous in the sensor, so any perturbation flips the faces whose
+    sensor sits at a threshold: a 1e-7 relative perturbation of the initial
+    velocities, given to the FP64 Newton run itself, flips faces and moves
+    the cells next to them by several K in the same way.  The flipped faces
+    and the cells excluded are counted and bounded (<= 10 % of this small
+    block, most of which is jet edge)."""
     import flame3d_amd.abi as abi
     import flame3d_amd.cases as cases
     import flame3d_amd.mech as mm
     import flame3d_amd.surrogate as sur
-    print(f"\n[jet] 3 steps FP32 + air surrogate vs FP64 + Newton, size={SIZE}", flush=True)
+    from helpers import branch_codes, flip_mask
+    print(f"\n[jet] 3 steps vs FP64 + Newton, size={SIZE}", flush=True)
     m = mm.load("air5")
     case = cases.reacting_jet(SIZE, m, dev="cpu", dt=2 * sur.AIR_DT)
-    res = {}
-    for prec, chem in (("fp64", abi.CHEM_NEWTON), ("fp32", abi.CHEM_SURROGATE)):
+
+    def run(prec, chem, perturb=0.0):
         ctx = abi.Context(*SIZE, m, prec, scheme=dict(recon_kind=abi.RECON_WENO7))
         if chem == abi.CHEM_SURROGATE:
             sur.attach(ctx, sur.AIR_PATH)
         ctx.set_metrics([x.cuda() for x in case.grid.metrics], case.grid.J.cuda())
         ctx.set_boundary(case.bc)
         Qg, rg = case.Q.cuda(), case.rhoi.cuda()
+        if perturb:
+            gen = torch.Generator(device="cuda").manual_seed(7)
+            for n in (1, 2, 3):
+                Qg[n] *= 1 + perturb * torch.randn(Qg[n].shape, generator=gen, device="cuda",
+                                                   dtype=torch.float64)
         Ug, Un, rn = ctx.zeros(5), ctx.zeros(5), ctx.zeros(m.Ns)
         ctx.prim2c(Ug, Qg)
+        codes = []
+
+        def half():
+            if chem == abi.CHEM_SURROGATE:
+                ctx.nn_chem(Ug, Qg, rg, case.dt / 2, 50, 1e-10, 1e-6)
+            else:
+                ctx.chem_newton(Ug, Qg, rg, case.dt / 2, 50, 1e-10, 1e-6)
+            ctx.fill(Qg, Ug, rg)
+        # f3d_time_step's sequence, with each stage's branch codes recorded
         for _ in range(3):
-            ctx.time_step(Ug, Un, rg, rn, Qg, case.dt, chem)
+            half()
+            for k in (1, 2, 3):
+                ctx.rk3_stage(k, Ug, Un, rg, rn, Qg, case.dt)
+                codes.append(branch_codes(ctx.workspace("phi").clone(), 1e-3, 0.1))
+            half()
         torch.cuda.synchronize()
         assert not ctx.nan_check(Ug)
-        res[prec] = (to_np(rg), to_np(Qg))
         ctx.close()
-    sl = (slice(4, -4),) * 3
-    Y = {p: res[p][0][sl] / res[p][1][sl + (0,)][..., None] for p in res}
-    dY = np.max(np.abs(Y["fp32"] - Y["fp64"]))
-    dTa = np.abs(res["fp32"][1][sl + (5,)] - res["fp64"][1][sl + (5,)])
-    mean, big = float(np.mean(dTa)), int(np.sum(dTa > 1.0))
-    print(f"  Y max diff {dY:.2e}  T max diff {dTa.max():.3f} K, mean {mean:.4f} K, "
-          f"{big} cells over 1 K", flush=True)
-    assert dY <= 1e-3
-    assert mean <= 0.02 and big <= 1e-3 * dTa.size
+        sl = (slice(4, -4),) * 3
+        return Qg[5][sl], rg[(slice(None),) + sl] / Qg[0][sl].unsqueeze(0), codes
+
+    T0, Y0, c0 = run("fp64", abi.CHEM_NEWTON)
+    cells = T0.numel()
+    for name, prec, chem, pert in (("fp64 + Newton, velocities x (1 + 1e-7 noise)", "fp64",
+                                    abi.CHEM_NEWTON, 1e-7),
+                                   ("fp64 stage + air surrogate", "fp64", abi.CHEM_SURROGATE, 0),
+                                   ("fp32 stage + Newton", "fp32", abi.CHEM_NEWTON, 0),
+                                   ("fp32 stage + air surrogate", "fp32", abi.CHEM_SURROGATE, 0)):
+        T, Y, c = run(prec, chem, pert)
+        dT, dY = (T - T0).abs(), float((Y - Y0).abs().max())
+        near, flips = flip_mask(c0, c, radius=2)
+        away = float(dT[~near].max()) if bool((~near).any()) else 0.0
+        print(f"  {name}: Y max diff {dY:.2e}, T max diff {float(dT.max()):.3f} K "
+              f"({int((dT > 1).sum())} cells over 1 K); {flips} face flips, "
+              f"{int(near.sum())} cells near them; T max diff away from them {away:.3f} K",
+              flush=True)
+        assert dY <= 1e-3, (name, dY)
+        assert away <= 1.0, (name, away)
+        assert int(near.sum()) <= 0.1 * cells, (name, int(near.sum()))
