@@ -539,7 +539,8 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), rad
                                  ("fp32_nn", "fp32", True, 0.0),
                                  ("fp64_perturbed", "fp64", False, 1e-7)):
         curve = {"step": [], "T_linf_K": [], "Y_linf": [], "cells_T_over_1K": [],
-                 "face_flips": [], "cells_near_flips": [], "T_linf_away_K": []}
+                 "face_flips": [], "cells_near_flips": [], "T_linf_away_K": [],
+                 "Y_linf_away": []}
         near = torch.zeros(N[::-1], dtype=torch.bool, device="cuda")
         nflips = [0]
 
@@ -561,16 +562,18 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), rad
             curve["face_flips"].append(nflips[0])
             curve["cells_near_flips"].append(int(m.sum()))
             curve["T_linf_away_K"].append(float(dT[~m].max()))
+            curve["Y_linf_away"].append(float((Y - Yr).abs().amax(0)[~m].max()))
         log(f"[bench] accuracy: {name}")
         run(prec, nn, cmp, pert)
         first = next((s for s, t, y in zip(curve["step"], curve["T_linf_K"], curve["Y_linf"])
                       if t > ACC_BAR["T_K"] or y > ACC_BAR["Y"]), None)
         first_away = next((s for s, t, y in zip(curve["step"], curve["T_linf_away_K"],
-                                                curve["Y_linf"])
+                                                curve["Y_linf_away"])
                            if t > ACC_BAR["T_K"] or y > ACC_BAR["Y"]), None)
         at = {str(s): {"T_linf_K": curve["T_linf_K"][i], "Y_linf": curve["Y_linf"][i],
                        "cells_T_over_1K": curve["cells_T_over_1K"][i],
                        "T_linf_away_from_flips_K": curve["T_linf_away_K"][i],
+                       "Y_linf_away_from_flips": curve["Y_linf_away"][i],
                        "face_flips": curve["face_flips"][i]}
               for i, s in enumerate(curve["step"]) if s in marks or s == steps}
         out[name] = {"at_step": at, "first_step_over_bar": first,
