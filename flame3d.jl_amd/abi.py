@@ -106,6 +106,7 @@ _PROTOS = {
     "f3d_nn_set_params": [_vp, _vp],
     "f3d_nn_get_params": [_vp, _vp],
     "f3d_nn_train_step": [_vp, _vp, _vp, C.c_int32, C.c_float, _vp, _vp],
+    "f3d_nn_train_step_static": [_vp, _vp, _vp, C.c_int32, C.c_float, _vp, _vp],
     "f3d_nn_grad": [_vp, _vp, _vp, C.c_int32, _vp, _vp],
     "f3d_nn_forward": [_vp, _vp, _vp, C.c_int32, _vp],
     "f3d_nn_chem": [_vp, _vp, _vp, _vp, C.c_double, C.c_int32, C.c_double, C.c_double, _vp,
@@ -417,6 +418,16 @@ class Context:
             lp = self._chk(loss, (1,), self.torch.float32, "loss")
         _rc(self.lib.f3d_nn_train_step(self.h, X.data_ptr(), Y.data_ptr(), B, lr, lp,
                                        self.stream()), "f3d_nn_train_step")
+
+    def nn_train_step_static(self, X, Y, lr, loss=None):
+        """nn_train_step on the library's static scratch (no device
+        allocation; batch <= 8192)."""
+        B = self._nn_batch(X, Y, self.Ns)
+        lp = None
+        if loss is not None:
+            lp = self._chk(loss, (1,), self.torch.float32, "loss")
+        _rc(self.lib.f3d_nn_train_step_static(self.h, X.data_ptr(), Y.data_ptr(), B, lr, lp,
+                                              self.stream()), "f3d_nn_train_step_static")
 
     def nn_grad(self, X, Y):
         """Gradient (NN_NPARAM floats) and loss of the batch, as a device tensor."""

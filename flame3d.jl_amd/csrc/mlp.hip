@@ -16,6 +16,7 @@
 // a contiguous 16-byte read.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 
@@ -840,6 +841,41 @@ static int nn_partials(f3d_ctx* c, long tiles) {
   return F3D_OK;
 }
 
+// Scratch of f3d_nn_train_step_static: the per-tile partial gradients and
+// losses of at most kStaticTiles tiles of 64 samples (batch <= 8192), 6.4 MB.
+// Every tile writes all of its NN_NPARAM + 1 slots, so nothing needs
+// clearing.  Each GPU holds its own copy until the process ends; host code
+// uses the device address hipGetSymbolAddress gives on that device, never
+// the array's name.
+constexpr int kStaticTiles = 128;
+__device__ __align__(256) float scratch[(size_t)kStaticTiles * (NN_NPARAM + 1)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (with the device current) and kept.
+static int static_scratch(float** out) {
+  int dev = -1;
+  int rc = hip_check(hipGetDevice(&dev), "f3d_nn_train_step_static: hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("f3d_nn_train_step_static: device ordinal beyond the scratch table");
+    return F3D_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                   "f3d_nn_train_step_static: hipGetSymbolAddress");
+    if (rc) return rc;
+    g_scratch_ptr[dev] = (float*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return F3D_OK;
+}
+
 void nn_free(f3d_ctx* c) {
   void* ptrs[] = {c->nn_master, c->nn_m1, c->nn_m2, c->nn_Wb, c->nn_Wtb, c->nn_partial,
                   c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal};
@@ -1047,6 +1083,30 @@ int f3d_nn_train_step(f3d_ctx* c, const float* X, const float* Y, int32_t B, flo
                      (int)tiles, c->nn_master, c->nn_m1, c->nn_m2, (__bf16*)c->nn_Wb,
                      (__bf16*)c->nn_Wtb, lr, (float)b1, (float)b2, 1e-8f, bc1, bc2, loss);
   return hip_check(hipGetLastError(), "f3d_nn_train_step");
+}
+
+int f3d_nn_train_step_static(f3d_ctx* c, const float* X, const float* Y, int32_t B, float lr,
+                             float* loss, hipStream_t s) {
+  int rc = nn_batch_args(c, X, Y, B, "f3d_nn_train_step_static");
+  if (rc) return rc;
+  const long tiles = B / 64;
+  if (tiles > kStaticTiles) {
+    set_error("f3d_nn_train_step_static: batch size above the static scratch's 8192 samples");
+    return F3D_ERR_ARG;
+  }
+  float* part = nullptr;
+  rc = static_scratch(&part);
+  if (rc) return rc;
+  c->nn_step += 1;
+  const double b1 = 0.9, b2 = 0.999;
+  const float bc1 = (float)(1.0 - std::pow(b1, (double)c->nn_step));
+  const float bc2 = (float)(1.0 - std::pow(b2, (double)c->nn_step));
+  hipLaunchKernelGGL(k_nn_train_tile, dim3(tiles), dim3(256), 0, s, X, Y, B, c->g.Ns + 2, c->g.Ns,
+                     (const __bf16*)c->nn_Wb, (const __bf16*)c->nn_Wtb, c->nn_master, part);
+  hipLaunchKernelGGL(k_nn_adam, dim3(nblocks(NN_NPARAM + 1)), dim3(kBlock), 0, s, part,
+                     (int)tiles, c->nn_master, c->nn_m1, c->nn_m2, (__bf16*)c->nn_Wb,
+                     (__bf16*)c->nn_Wtb, lr, (float)b1, (float)b2, 1e-8f, bc1, bc2, loss);
+  return hip_check(hipGetLastError(), "f3d_nn_train_step_static");
 }
 
 int f3d_nn_forward(f3d_ctx* c, const float* X, float* Y, int32_t B, hipStream_t s) {

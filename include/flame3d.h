@@ -267,6 +267,17 @@ int f3d_nn_get_params(f3d_ctx* ctx, float* params);
  * loss (device float, may be NULL) receives sum (y - t)^2 / (B * 9). */
 int f3d_nn_train_step(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float lr,
                       float* loss, hipStream_t s);
+/* f3d_nn_train_step without any device allocation (no counterpart in the
+ * reference: for callers that cannot allocate, such as code under HIP graph
+ * capture or a training loop that forbids hipMalloc).  The per-tile partial
+ * gradients live in a static device array of the library (one per GPU, 6.4 MB,
+ * held until the process ends) instead of the context's buffer that
+ * f3d_nn_train_step allocates on first use or growth: B <= 8192 (B % 64 == 0),
+ * else F3D_ERR_ARG.  Every call on a GPU uses that GPU's one array, so calls
+ * that overlap on one GPU (two streams, two contexts) race; calls in stream
+ * order do not.  Same kernels and result as f3d_nn_train_step. */
+int f3d_nn_train_step_static(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float lr,
+                             float* loss, hipStream_t s);
 /* Gradient of that loss w.r.t. the packed parameters (device, F3D_NN_NPARAM
  * floats, followed by the loss) without updating the weights. */
 int f3d_nn_grad(f3d_ctx* ctx, const float* X, const float* Y, int32_t B, float* grad,
