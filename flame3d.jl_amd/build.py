@@ -10,6 +10,9 @@ parallel), then linked into flame3d.jl_amd/libflame3d.so.  Flags:
   -fno-hip-fp32-correctly-rounded-divide-sqrt
                                    hardware float div/sqrt on the FP32 path
                                    (double div/sqrt stay IEEE)
+  chem.hip also: -ffp-contract=off  the FP64 chemistry rounds every product
+                                   and sum as the oracle does (with its own
+                                   exp/log), so the two agree bit for bit
 """
 from __future__ import annotations
 
@@ -32,6 +35,9 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Wno-unused-variable"]
 
 
+FILE_FLAGS = {"chem.hip": ["-ffp-contract=off"]}
+
+
 def build(jobs: int = 8, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -41,7 +47,7 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     for src in srcs:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+        cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         while len([p for p in procs if p[0].poll() is None]) >= jobs:

@@ -7,6 +7,70 @@
 
 namespace f3d {
 
+// exp and log of the chemistry, written with +, -, *, /, explicit fused
+// multiply-adds (correctly rounded on any IEEE-754 machine) and exact binary
+// scaling only, so that a CPU and this GPU (chem.hip is compiled without
+// implicit contraction of a*b+c) produce the same bits; the oracle restates
+// them.
+// exp: Cody and Waite's argument reduction (W. J. Cody and W. Waite,
+// "Software Manual for the Elementary Functions", 1980): x = k ln2 + r with
+// |r| <= ln2/2 and ln2 split in a 32-bit head (k * head exact for |k| < 2^11)
+// and a tail; exp(r) by its Taylor series to r^13 (remainder < 5e-18
+// relative) in Horner form with fma; 2^k by ldexp.  log: x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1)/(m + 1) (|s| <= 0.172),
+// by its series to s^21 (remainder < 1e-18).
+__device__ inline double f3d_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return __builtin_inf();
+  if (x < -745.1332191019412) return 0.0;
+  const double kd = floor(x * 1.4426950408889634 + 0.5);
+  const double r = (x - kd * 6.93147180369123816490e-01) - kd * 1.90821492927058770002e-10;
+  double p = 1.0 / 6227020800.0;  // 1/13!
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)kd);
+}
+
+__device__ inline double f3d_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
+  if (x == __builtin_inf()) return x;
+  int e = 0;
+  double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  p = fma(p, z, 1.0);
+  const double ed = (double)e;
+  return ed * 6.93147180369123816490e-01 + (ed * 1.90821492927058770002e-10 + (2.0 * s) * p);
+}
+
+__device__ inline double f3d_log10(double x) { return f3d_log(x) * 0.43429448190325182765; }
+__device__ inline double f3d_exp10(double y) { return f3d_exp(y * 2.30258509299404568402); }
+
 __device__ inline double ipow(double x, int n) {
   // Julia Float64^Int for the stoichiometric/order exponents that occur (|n| <= 3)
   if (n == 0) return 1.0;
@@ -45,7 +109,7 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 template <int NS, bool JAC, bool FULL = false, int NA = NS>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
                                        double (&A)[NA][NA], double* wdot) {
-  const double lgT = log(T);
+  const double lgT = f3d_log(T);
   const double invT = 1.0 / T;
   double gi[NS];
   gibbs<NS>(P, T, lgT, invT, gi);
@@ -60,11 +124,11 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
   }
   const int Nr = P->Nr;
   for (int n = 0; n < Nr; ++n) {
-    const double kf = P->Arr[n][0] * exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+    const double kf = P->Arr[n][0] * f3d_exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
     double dg = 0;
 #pragma unroll
     for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-    const double Kc = ipow(RsT, P->sgm[n]) * exp(dg);
+    const double Kc = ipow(RsT, P->sgm[n]) * f3d_exp(dg);
     double q_f = kf;
     double q_r = q_f / Kc;
 #pragma unroll
@@ -88,15 +152,16 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       double mix = 0.0;
 #pragma unroll
       for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
-      const double redP = mix / kf * P->loP[n][0] * exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
-      const double logPred = log10(redP);
+      const double redP = mix / kf * P->loP[n][0] * f3d_exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      const double logPred = f3d_log10(redP);
       const double A_ = P->Troe[n][0];
-      const double logFcent = log10((1 - A_) * exp(-T / P->Troe[n][1]) +
-                                    A_ * exp(-T / P->Troe[n][2]) + exp(-P->Troe[n][3] * invT));
+      const double logFcent = f3d_log10((1 - A_) * f3d_exp(-T / P->Troe[n][1]) +
+                                        A_ * f3d_exp(-T / P->Troe[n][2]) +
+                                        f3d_exp(-P->Troe[n][3] * invT));
       const double troe_c = -0.4 - 0.67 * logFcent;
       const double troe_n = 0.75 - 1.27 * logFcent;
       const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
-      const double F_troe = pow(10.0, logFcent / (1.0 + troe * troe));
+      const double F_troe = f3d_exp10(logFcent / (1.0 + troe * troe));
       const double k_1 = (redP / (1.0 + redP)) * F_troe;
       q_f *= k_1;
       q_r *= k_1;
@@ -174,7 +239,7 @@ template <int NS>
 __device__ inline void production_rate_block3(const DevParams* __restrict__ P, const double* sc,
                                               double T, const int (&ci)[3], double* wdot,
                                               double (&M)[3][3]) {
-  const double lgT = log(T);
+  const double lgT = f3d_log(T);
   const double invT = 1.0 / T;
   double gi[NS], cc[3];
   gibbs<NS>(P, T, lgT, invT, gi);
@@ -192,11 +257,11 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
   }
   const int Nr = P->Nr;
   for (int n = 0; n < Nr; ++n) {
-    const double kf = P->Arr[n][0] * exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+    const double kf = P->Arr[n][0] * f3d_exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
     double dg = 0;
 #pragma unroll
     for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-    const double Kc = ipow(RsT, P->sgm[n]) * exp(dg);
+    const double Kc = ipow(RsT, P->sgm[n]) * f3d_exp(dg);
     const double kr = kf / Kc;
     // q_f, q_r: the full mass-action products (same order as production_rate,
     // so wdot is bit-identical to it); o_f, o_r: the products over the
@@ -238,15 +303,16 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
       double mix = 0.0;
 #pragma unroll
       for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
-      const double redP = mix / kf * P->loP[n][0] * exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
-      const double logPred = log10(redP);
+      const double redP = mix / kf * P->loP[n][0] * f3d_exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      const double logPred = f3d_log10(redP);
       const double A_ = P->Troe[n][0];
-      const double logFcent = log10((1 - A_) * exp(-T / P->Troe[n][1]) +
-                                    A_ * exp(-T / P->Troe[n][2]) + exp(-P->Troe[n][3] * invT));
+      const double logFcent = f3d_log10((1 - A_) * f3d_exp(-T / P->Troe[n][1]) +
+                                        A_ * f3d_exp(-T / P->Troe[n][2]) +
+                                        f3d_exp(-P->Troe[n][3] * invT));
       const double troe_c = -0.4 - 0.67 * logFcent;
       const double troe_n = 0.75 - 1.27 * logFcent;
       const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
-      const double F_troe = pow(10.0, logFcent / (1.0 + troe * troe));
+      const double F_troe = f3d_exp10(logFcent / (1.0 + troe * troe));
       const double k_1 = (redP / (1.0 + redP)) * F_troe;
       q_f *= k_1;
       q_r *= k_1;

@@ -182,30 +182,93 @@ D internal_energy_t(const Grid& g, const or_thermo* th, D T, const D* rhoi) {
   return result * th->Ru;
 }
 
+// exp and log of the chemistry: the library's f3d_exp / f3d_log
+// (flame3d.jl_amd/csrc/f3d_chem.h) restated with the same constants and
+// operation order, so that the oracle's chemistry and the GPU's agree bit
+// for bit (neither contracts a*b+c implicitly; the polynomials use explicit,
+// correctly rounded fma).  exp: Cody and Waite's reduction
+// x = k ln2 + r (ln2 split so k * head is exact), Taylor series of exp(r) to
+// r^13, ldexp; log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), 2 atanh((m-1)/(m+1))
+// by its series to s^21.  Accuracy against libm: tests/.
+double dexp(double x) {
+  if (x != x) return x;
+  if (x > 709.782712893384) return HUGE_VAL;
+  if (x < -745.1332191019412) return 0.0;
+  const double kd = std::floor(x * 1.4426950408889634 + 0.5);
+  const double r = (x - kd * 6.93147180369123816490e-01) - kd * 1.90821492927058770002e-10;
+  double p = 1.0 / 6227020800.0;
+  p = std::fma(p, r, 1.0 / 479001600.0);
+  p = std::fma(p, r, 1.0 / 39916800.0);
+  p = std::fma(p, r, 1.0 / 3628800.0);
+  p = std::fma(p, r, 1.0 / 362880.0);
+  p = std::fma(p, r, 1.0 / 40320.0);
+  p = std::fma(p, r, 1.0 / 5040.0);
+  p = std::fma(p, r, 1.0 / 720.0);
+  p = std::fma(p, r, 1.0 / 120.0);
+  p = std::fma(p, r, 1.0 / 24.0);
+  p = std::fma(p, r, 1.0 / 6.0);
+  p = std::fma(p, r, 0.5);
+  p = std::fma(p, r, 1.0);
+  p = std::fma(p, r, 1.0);
+  return std::ldexp(p, (int)kd);
+}
+
+double dlog(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : std::nan("");
+  if (x == HUGE_VAL) return x;
+  int e = 0;
+  double m = std::frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 21.0;
+  p = std::fma(p, z, 1.0 / 19.0);
+  p = std::fma(p, z, 1.0 / 17.0);
+  p = std::fma(p, z, 1.0 / 15.0);
+  p = std::fma(p, z, 1.0 / 13.0);
+  p = std::fma(p, z, 1.0 / 11.0);
+  p = std::fma(p, z, 1.0 / 9.0);
+  p = std::fma(p, z, 1.0 / 7.0);
+  p = std::fma(p, z, 1.0 / 5.0);
+  p = std::fma(p, z, 1.0 / 3.0);
+  p = std::fma(p, z, 1.0);
+  const double ed = (double)e;
+  return ed * 6.93147180369123816490e-01 + (ed * 1.90821492927058770002e-10 + (2.0 * s) * p);
+}
+
+inline double xexp(double x) { return dexp(x); }
+inline double xlog(double x) { return dlog(x); }
+inline double xlog10(double x) { return dlog(x) * 0.43429448190325182765; }
+inline double xexp10(double y) { return dexp(y * 2.30258509299404568402); }
+// the counting type tallies each as one transcendental
+inline CR xexp(CR a) { return exp(a); }
+inline CR xlog(CR a) { return log(a); }
+inline CR xlog10(CR a) { return log10(a); }
+inline CR xexp10(CR a) { return pow(CR(10.0), a); }
+
 // vproductionRate / vproductionRate_Jac, reactions.jl:311-460 (see or_production_rate)
 template <class D>
 void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D* sc, D T,
              double jac_eps, int32_t full_jac, D* wdot, D* Arate) {
-  using std::exp;
-  using std::log;
-  using std::log10;
-  using std::pow;
   const int Ns = g.Ns, Nr = g.Nr;
   std::vector<D> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
   for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
   if (Arate)
     for (int m = 0; m < Ns * Ns; ++m) Arate[m] = 0.0;
-  D lgT = log(T);
+  D lgT = xlog(T);
   D invT = 1.0 / T;
   for (int n = 0; n < Nr; ++n)
-    kf[n] = kin->Arr[3 * n] * exp(kin->Arr[3 * n + 1] * lgT - kin->Arr[3 * n + 2] * invT);
+    kf[n] = kin->Arr[3 * n] * xexp(kin->Arr[3 * n + 1] * lgT - kin->Arr[3 * n + 2] * invT);
   gibbs_t<D>(g, th, T, lgT, invT, gi.data());
   D RsT = th->Ru / kin->atm * 1e6 * T;
   for (int n = 0; n < Nr; ++n) {
     D dg = 0;
     for (int m = 0; m < Ns; ++m)
       dg += gi[m] * (double)(kin->vf[m + Ns * n] - kin->vr[m + Ns * n]);
-    Kc[n] = ipow_t(RsT, kin->sgm[n]) * exp(dg);
+    Kc[n] = ipow_t(RsT, kin->sgm[n]) * xexp(dg);
   }
   for (int n = 0; n < Nr; ++n) {
     D q_f = kf[n];
@@ -228,15 +291,15 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
       for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
       const double* lp = kin->loP + 3 * n;
       const double* tr = kin->Troe + 4 * n;
-      D redP = mixture / kf[n] * lp[0] * exp(lp[1] * lgT - lp[2] * invT);
-      D logPred = log10(redP);
+      D redP = mixture / kf[n] * lp[0] * xexp(lp[1] * lgT - lp[2] * invT);
+      D logPred = xlog10(redP);
       double A = tr[0];
       D logFcent =
-          log10((1 - A) * exp(-T / tr[1]) + A * exp(-T / tr[2]) + exp(-tr[3] * invT));
+          xlog10((1 - A) * xexp(-T / tr[1]) + A * xexp(-T / tr[2]) + xexp(-tr[3] * invT));
       D troe_c = -0.4 - 0.67 * logFcent;
       D troe_n = 0.75 - 1.27 * logFcent;
       D troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
-      D F_troe = pow(D(10.0), logFcent / (1.0 + troe * troe));
+      D F_troe = xexp10(logFcent / (1.0 + troe * troe));
       D k_1 = (redP / (1.0 + redP)) * F_troe;
       q_f *= k_1;
       q_r *= k_1;
@@ -432,9 +495,12 @@ static void gibbs_impl(const Grid& g, const or_thermo* th, double T, double lgT,
   gibbs_t<double>(g, th, T, lgT, invT, gi);
 }
 
+double or_exp(double x) { return dexp(x); }
+double or_log(double x) { return dlog(x); }
+
 void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g) {
   Grid gr(d);
-  gibbs_impl(gr, th, T, std::log(T), 1.0 / T, g);
+  gibbs_impl(gr, th, T, dlog(T), 1.0 / T, g);
 }
 
 // CV, thermo.jl:194-203 (J/m^3/K)

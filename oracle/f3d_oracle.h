@@ -54,6 +54,9 @@ double or_get_T_gap(const or_dims* d, const or_thermo* th, double e, const doubl
 double or_get_T_from(const or_dims* d, const or_thermo* th, double e, const double* rhoi,
                      double T_guess, double tol, int32_t maxiter, double tmin, double tmax);
 void or_h_specs(const or_dims* d, const or_thermo* th, double T, double* h);
+/* exp / log of the chemistry rates (the library's f3d_exp / f3d_log restated) */
+double or_exp(double x);
+double or_log(double x);
 void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g);
 void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
                         const double* sc, double T, double jac_eps, int32_t full_jac,

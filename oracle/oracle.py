@@ -99,6 +99,10 @@ class Oracle:
                               int(s.get("sensor_kind", 0)), int(s.get("recon_kind", 0)))
         L = self.lib
         L.or_get_T.restype = C.c_double
+        L.or_exp.restype = C.c_double
+        L.or_exp.argtypes = [C.c_double]
+        L.or_log.restype = C.c_double
+        L.or_log.argtypes = [C.c_double]
         L.or_get_T_from.restype = C.c_double
         L.or_weno7z.restype = C.c_double
         L.or_internal_energy.restype = C.c_double

@@ -591,3 +591,38 @@ def test_newton_accepts_nasa7_breakpoint_gap(h2):
         it = orc.chem_newton(U, Q, R, 1e-7)
         assert 1 <= int(it.ravel()[0]) <= 4, it.ravel()[0]
         assert abs(Q[4, 4, 4, 5] - 1000.0) < 1.0  # T from GetT stays at the stream's 1000 K
+
+
+def test_chemistry_exp_log_accuracy(h2):
+    """The exp / log the chemistry rates use on the CPU and the GPU (restated
+    from the library's f3d_exp / f3d_log, only +, -, *, / and exact scaling):
+    within 1 ulp (exp) and 3 ulp (log) of the correctly rounded value over
+    the arguments the kinetics produce (measured maxima), exact where the
+    result is exact, IEEE special values."""
+    from oracle import Oracle
+    L = Oracle(1, 1, 1, h2).lib
+    rng = np.random.default_rng(8)
+    xs = np.concatenate([rng.uniform(-700, 700, 20000), rng.uniform(-50, 50, 20000),
+                         rng.uniform(-1, 1, 5000), [-745.0, 709.7, 1e-300, -1e-300]])
+    got = np.array([L.or_exp(float(x)) for x in xs])
+    ref = np.exp(xs.astype(np.longdouble))
+    ulp = np.spacing(ref.astype(np.float64))
+    err = np.abs(got - ref.astype(np.float64))
+    ok = ref > np.finfo(np.float64).tiny  # normal results
+    assert np.max(err[ok] / ulp[ok]) <= 1.0
+    ys = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.5, 2.0, 20000),
+                         rng.uniform(200, 6000, 5000), [5e-324, 1.7e308]])
+    got = np.array([L.or_log(float(y)) for y in ys])
+    ref = np.log(ys.astype(np.longdouble)).astype(np.float64)
+    ulp = np.spacing(np.abs(ref))
+    nz = np.abs(ref) > 1e-300
+    assert np.max(np.abs(got - ref)[nz] / ulp[nz]) <= 3.0
+    assert L.or_exp(0.0) == 1.0 and L.or_log(1.0) == 0.0
+    for k in (-1000, -3, 1, 52, 1000):
+        # the argument k ln2 is itself rounded: relative |k ln2| 2^-53 in the result
+        assert L.or_exp(k * math.log(2.0)) == pytest.approx(
+            2.0 ** k, rel=abs(k) * math.log(2.0) * 2.3e-16 + 4e-16)
+        assert L.or_log(2.0 ** k) == pytest.approx(k * math.log(2.0), rel=2e-16, abs=0)
+    assert math.isnan(L.or_exp(float("nan"))) and math.isnan(L.or_log(-1.0))
+    assert L.or_exp(800.0) == math.inf and L.or_exp(-800.0) == 0.0
+    assert L.or_log(0.0) == -math.inf and L.or_log(math.inf) == math.inf
