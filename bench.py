@@ -381,7 +381,7 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
             nfb.append(ctx.nn_chem(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6))
             parts.append(ctx.nn_timing())
         ctx.nn_timing(False)
-        extra["nn_parts_ms"] = [sum(p[k] for p in parts) / len(parts) for k in range(3)]
+        extra["nn_parts_ms"] = [sum(p[k] for p in parts) / len(parts) for k in range(4)]
         extra["nn_parts_fallback"] = sum(nfb) / len(nfb)
         del Uc, Qc, rc
     elif chemistry:
@@ -654,6 +654,19 @@ def cpu_baseline_reference(t_end="4e-3"):
                       f"1e-8 s in {dt:.1f} s, per-step output discarded"}
 
 
+def stage_roofline(kt, Ns):
+    """HBM roofline of the stage kernel that takes the most time per stage
+    (ms x launches) in the per-kernel table `kt` of one precision (traffic:
+    the counters are measured on the reacting mixing layer only, so None)."""
+    share = {k: v["ms"] * v["launches_per_stage"] for k, v in kt.items()}
+    dom = max(share, key=share.get)
+    dk = kt[dom]
+    return {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dk["GBps"] / HBM_PEAK_GBS,
+            "traffic": None, "alg_bytes_per_launch": dk["alg_bytes"],
+            "avg_launch_ms": dk["ms"], "ms_per_stage": share[dom]}
+
+
 def bench_air(a, torch, abi, cases, mm):
     """The shock-laden mixing layer of 5-species air, non-reacting, hybrid
     WENO5-Z / UP7 switched by the Ducros sensor at 0.1: one RK3 stage (the
@@ -687,6 +700,7 @@ def bench_air(a, torch, abi, cases, mm):
         "vs_baseline": None, "dtype": "f32 flux, f64 state", "data": "synthetic",
         "config": {"workload": f"shock-laden mixing layer {N[0]}^3, 5-species air, non-reacting, "
                                "WENO5-Z/UP7 switched by the Ducros sensor at 0.1"},
+        "roofline": stage_roofline(ktab["fp32"], mech.Ns),
         "fp64": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64},
         "speedup_fp32_vs_fp64": ms64 / ms32, "error_fp32_vs_fp64": errs, "kernels": ktab}))
     c32.close()
@@ -727,6 +741,13 @@ def bench_jet(a, torch, abi, cases, mm):
     log(f"[bench] FP32 + NN {msnn:.3f} ms/stage (chemistry {rnn['chem_ms_mean']:.2f} ms/call, "
         f"fallback {fb:.4f})")
     names = ["rho", "u", "v", "w", "p", "T", "e"]
+    npts = (N[0] + 8) * (N[1] + 8) * (N[2] + 8)
+    nf = (N[0] + 1) * N[1] * N[2]
+    ktab = {}
+    for name, (ms, per) in rnn["kernels"].items():
+        b = kernel_bytes(name, 4, mech.Ns, npts, nf, nc)
+        ktab[name] = {"ms": ms, "launches_per_stage": per, "alg_bytes": b,
+                      "GBps": b / (ms * 1e-3) / 1e9, "frac_hbm": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     print(json.dumps({
         "metric": "cell-updates/s per RK stage, FP32+NN vs FP64 plain (reacting air jet)",
         "value": nc / (msnn * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
@@ -744,7 +765,8 @@ def bench_jet(a, torch, abi, cases, mm):
                     "fallback_fraction": fb},
         "speedup_vs_fp64_plain": ms64 / msnn, "speedup_fp32_newton_vs_fp64": ms64 / ms32,
         "error_vs_fp64_plain": state_errors(rnn, r64, mech),
-        "error_fields_vs_fp64_plain": field_errors(torch, rnn["Q"], r64["Q"], names)}))
+        "error_fields_vs_fp64_plain": field_errors(torch, rnn["Q"], r64["Q"], names),
+        "roofline": stage_roofline(ktab, mech.Ns), "kernels": {"fp32": ktab}}))
 
 
 def main():
@@ -835,7 +857,8 @@ def main():
     # (profiles/chem_opcount.json) over the event-timed launches
     ops = json.load(open(OPCOUNT_FILE))["mechanisms"]["h2_lidryer"]
     hot = int((case.Q[5][(slice(4, -4),) * 3] > 500.0).sum())
-    nn_ms, nn_scan_ms, nn_fb_ms = r32["nn_parts_ms"]
+    net_ms, epi_ms, nn_scan_ms, nn_fb_ms = r32["nn_parts_ms"]
+    nn_ms = net_ms + epi_ms
     nn_f64 = hot * ops["surrogate_epilogue"]["flop"]
     nn_bf16 = hot * MLP_FLOP_PER_CELL
     chem_tab = {

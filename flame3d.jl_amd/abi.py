@@ -531,12 +531,12 @@ class Context:
 
     def nn_timing(self, enable=None):
         """Enable (True) / disable (False) event timing of f3d_nn_chem; with
-        enable None return the last timed call's [network kernel, scan +
-        compaction, read-back + Newton fallback] in ms."""
+        enable None return the last timed call's [network kernel, gate
+        epilogue kernel, scan + compaction, read-back + Newton fallback] in ms."""
         if enable is not None:
             _rc(self.lib.f3d_nn_timing(self.h, int(bool(enable)), None), "f3d_nn_timing")
             return None
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 4)()
         _rc(self.lib.f3d_nn_timing(self.h, 1, ms), "f3d_nn_timing")
         return list(ms)
 

@@ -412,7 +412,7 @@ def reacting_mixing_layer(N=(256, 256, 256), mech=None, seed=12345, dev="cuda", 
     return Case("reacting_mixing_layer", grid, Q.contiguous(), rhoi.contiguous(), bc, dt)
 
 
-def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0, D=0.01,
+def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=1.0, D=0.01,
                  Ma=1.5, T_jet=5000.0, T_air=300.0, P0=3596.0, noise=0.02, cfl=0.5,
                  dt=None) -> Case:
     """Supersonic hot-air jet, the coupled reacting workload of 5-species air:
@@ -426,7 +426,10 @@ def reacting_jet(N=(512, 256, 256), mech=None, seed=12345, dev="cuda", beta=2.0,
     x fixed inflow ghosts / zero-gradient outflow, y and z zero gradient.
     The time step is the CFL step unless `dt` fixes it (a surrogate trained
     for one chemistry step needs a fixed one; it must not exceed the CFL
-    step)."""
+    step).  At 512x256x256 and the fixed 1.2e-8 s step, beta = 2 (the
+    smallest y cells on the axis, convective CFL 0.44) runs away on the jet
+    axis after ~15 steps (T 8200 K, then NaN at step 20, in FP64); beta = 1
+    (CFL 0.23) holds over 50 steps."""
     from . import abi
     Nx, Ny, Nz = N
     grid = stretched_y_grid(Nx, Ny, Nz, 20 * D, 10 * D, 10 * D, beta=beta, dev=dev)

@@ -89,11 +89,13 @@ struct f3d_ctx {
   int32_t* nn_fbcount = nullptr;  // rejected cells per 64-cell tile, then their offsets
   int32_t* nn_fblist = nullptr;   // rejected cell ids in cell order [Nx*Ny*Nz]
   int32_t* nn_fbtotal = nullptr;  // number of rejected cells
+  float* nn_out = nullptr;        // the network's raw outputs [Ns][Nx*Ny*Nz] for the epilogue
   // f3d_nn_timing: events around the last f3d_nn_chem's network kernel, its
-  // scan / compaction and its Newton fallback, on the call's stream
+  // gate epilogue kernel, its scan / compaction and its Newton fallback, on
+  // the call's stream
   bool nn_timing = false;
   bool nn_timed = false;
-  hipEvent_t nn_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t nn_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace f3d {

@@ -280,10 +280,14 @@ __global__ void __launch_bounds__(256)
 // the FP32 outputs: a layer's 4x4 (or 4x2) 16x16 output tiles stay in
 // registers until every lane has read the previous activations.
 // ---------------------------------------------------------------------------
+// Rows are padded by 16 bytes (one float for Y): a 16-byte MFMA operand read
+// by the 16 lanes of a quarter-wave then spans different bank groups instead
+// of the same one (an unpadded row of 64 bf16 is 128 B, a whole bank cycle).
+constexpr int kRowPad = 8;
 union FwdLds {
-  __bf16 X[64][NN_INP];
-  __bf16 A[64][NN_H];
-  float Y[64][NN_OUTP];
+  __bf16 X[64][NN_INP + kRowPad];
+  __bf16 A[64][NN_H + kRowPad];
+  float Y[64][NN_OUTP + 1];
 };
 
 // one dense layer for the wave's 64 samples: NI inputs, NO outputs (compile-time)
@@ -432,14 +436,16 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
 }
 
 // ---------------------------------------------------------------------------
-// Fused surrogate chemistry step (pre_input -> network -> post_predict,
-// reactions.jl:2-56): one wave = 64 interior cells.  The network runs for all
-// 64 lanes (uniform control flow around the MFMAs); each active lane then
-// applies seven gates and either commits the prediction, with the FP64 energy
-// bookkeeping of the backward-Euler integrator (chem_epilogue), or marks its
-// cell for the Newton pass and leaves it untouched (status = 1 + gate); the
-// marked cells are then listed in cell order (k_nn_scan, k_nn_compact):
-//   1. Mahalanobis distance of the normalized inputs <= d_max (out of distribution);
+// Surrogate chemistry step (pre_input -> network -> post_predict,
+// reactions.jl:2-56) in two kernels: k_nn_net runs the network on the matrix
+// cores, one wave per 64 interior cells, and stores its raw outputs; then
+// k_nn_epi, one thread per cell with the whole register budget for the FP64
+// work, applies the gates and either commits the prediction, with the FP64
+// energy bookkeeping of the backward-Euler integrator, or marks its cell for
+// the Newton pass and leaves it untouched (status = 1 + gate); the marked
+// cells are then listed in cell order (k_nn_scan, k_nn_compact):
+//   1. Mahalanobis distance of the normalized inputs <= d_max (out of
+//      distribution; in k_nn_net, on the inputs it already holds);
 //   2. element balance of the raw prediction, each element's net change against
 //      the gross elemental turnover (then projected to conserve elements);
 //   3. positivity, Y_k + dY_k >= -neg_tol;
@@ -454,24 +460,23 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
 //   7. chain explosiveness: every eigenvalue of dt d wdot / d c over the chain
 //      carriers below chain_max (near dt lambda = 1 a hot radical-free mixture
 //      has a near-root with a tiny residual while the Newton step ignites it).
+// Between the two kernels the outputs take 4 B per cell and output
+// ([Ns][cells], 0.6 GB at 256^3 for Ns = 9, written and read once).
 // ---------------------------------------------------------------------------
 template <int NS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-    k_nn_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
-              const int64_t* __restrict__ tag, const DevParams* __restrict__ P,
-              const NNDev* __restrict__ C, const __bf16* __restrict__ Wb,
-              const float* __restrict__ master, Geom g, double dt,
-              int32_t* __restrict__ tile_count, int8_t* __restrict__ status) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+    k_nn_net(const double* __restrict__ Q, const double* __restrict__ rhoi,
+             const int64_t* __restrict__ tag, const DevParams* __restrict__ P,
+             const NNDev* __restrict__ C, const __bf16* __restrict__ Wb,
+             const float* __restrict__ master, Geom g, float* __restrict__ out,
+             int8_t* __restrict__ status) {
   constexpr int NIN = NS + 2;  // T, log P, Box-Cox of each species
   __shared__ __align__(16) FwdLds L;
   const int lane = threadIdx.x;
   const long ncell = (long)g.Nx * g.Ny * g.Nz;
-  const unsigned tile = xcd_block();
-  const long id = (long)tile * 64 + lane;
+  const long id = (long)xcd_block() * 64 + lane;
   const long NP = g.NP;
-  long p = 0;
   bool act = false;
-  double r[NS], rho = 0.0, T = 0.0;
   float xn[NIN];
 #pragma unroll
   for (int k = 0; k < NIN; ++k) xn[k] = 0.0f;
@@ -479,39 +484,33 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     const int a = (int)(id % g.Nx);
     const int t = (int)(id / g.Nx);
     const int b = t % g.Ny, c = t / g.Ny;
-    p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
-    T = Q[p + 5 * NP];
+    const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+    const double T = Q[p + 5 * NP];
     act = (!tag || tag[p] == 0) && (T > P->T_criteria);
     status[id] = 0;
+    if (act) {
+      // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
+      const double rinv = 1.0 / fmax(Q[p], kEps);
+      const float lam = C->boxcox_lambda;
+      float x[NIN];
+      x[0] = (float)T;
+      x[1] = logf((float)Q[p + 4 * NP]);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const float Y = (float)fmax(rhoi[p + n * NP] * rinv, (double)C->y_floor);
+        x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
+      }
+#pragma unroll
+      for (int k = 0; k < NIN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
+    }
   }
   // a tile with no reacting cell (cold gas, tagged cells) skips the network;
   // the workgroup is one wave, so the exit is uniform over the workgroup
-  if (!__any(act)) {
-    if (lane == 0) tile_count[tile] = 0;
-    return;
-  }
-  if (act) {
-    // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
-    rho = Q[p];
-    const double rinv = 1.0 / fmax(rho, kEps);
-    const float lam = C->boxcox_lambda;
-    float x[NIN];
-    x[0] = (float)T;
-    x[1] = logf((float)Q[p + 4 * NP]);
-#pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      r[n] = rhoi[p + n * NP];
-      const float Y = (float)fmax(r[n] * rinv, (double)C->y_floor);
-      x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
-    }
-#pragma unroll
-    for (int k = 0; k < NIN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
-  }
+  if (!__any(act)) return;
 #pragma unroll
   for (int k = 0; k < NN_INP; ++k) L.X[lane][k] = (__bf16)(k < NIN ? xn[k] : 0.0f);
   __syncthreads();
   nn_forward_wave(L, Wb, master, lane);
-  int code = act ? 1 : 0;  // 1 accepted, 2..8 the first gate that failed
   if (act) {
     // 1. Mahalanobis distance of the normalized inputs
     float d2 = 0.0f;
@@ -522,7 +521,32 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       for (int j = 0; j < NIN; ++j) row += C->mahal_inv[i][j] * (xn[j] - C->mahal_mean[j]);
       d2 += (xn[i] - C->mahal_mean[i]) * row;
     }
-    if (!(sqrtf(fmaxf(d2, 0.0f)) <= C->d_max)) code = 2;
+    status[id] = (int8_t)((sqrtf(fmaxf(d2, 0.0f)) <= C->d_max) ? 1 : 2);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) out[(long)n * ncell + id] = L.Y[lane][n];
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+    k_nn_epi(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
+             const DevParams* __restrict__ P, const NNDev* __restrict__ C, Geom g, double dt,
+             const float* __restrict__ out, int32_t* __restrict__ tile_count,
+             int8_t* __restrict__ status) {
+  const long ncell = (long)g.Nx * g.Ny * g.Nz;
+  const long id = gid();  // a wave holds the 64 cells of one tile
+  const long NP = g.NP;
+  int code = (id < ncell) ? status[id] : 0;  // 0 inactive, 1 candidate, 2 Mahalanobis
+  if (code == 1) {
+    const int a = (int)(id % g.Nx);
+    const int t = (int)(id / g.Nx);
+    const int b = t % g.Ny, c = t / g.Ny;
+    const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+    const double T = Q[p + 5 * NP];
+    const double rho = Q[p];
+    double r[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) r[n] = rhoi[p + n * NP];
     // post_predict (reactions.jl:26-56) on the de-normalized outputs
     const double rinv = 1.0 / fmax(rho, kEps);
     const double lam = (double)C->boxcox_lambda;
@@ -530,7 +554,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       Y[n] = fmax(r[n] * rinv, 0.0);
-      const double o = (double)(L.Y[lane][n] * C->out_std[n] + C->out_mean[n]);
+      const double o = (double)(out[(long)n * ncell + id] * C->out_std[n] + C->out_mean[n]);
       if (C->label_mode == 0) {
         dY0[n] = o;
       } else if (C->label_mode == 2) {
@@ -673,12 +697,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     }
     status[id] = (int8_t)code;
   }
-  const bool ok = (code == 1);
   // rejected cells of this 64-cell tile (status >= 2); the ordered list the
   // Newton pass runs over is built from these counts (k_nn_scan, k_nn_compact)
-  const bool rej = act && !ok;  // code 2..8
-  const unsigned long long bal = __ballot(rej);
-  if (lane == 0) tile_count[tile] = (int32_t)__popcll(bal);
+  const unsigned long long bal = __ballot(code >= 2);
+  if ((threadIdx.x & 63) == 0 && id < ncell) tile_count[id >> 6] = (int32_t)__popcll(bal);
 }
 
 // Exclusive scan of n counts in place (one workgroup of 1024 threads, each
@@ -878,7 +900,8 @@ static int static_scratch(float** out) {
 
 void nn_free(f3d_ctx* c) {
   void* ptrs[] = {c->nn_master, c->nn_m1, c->nn_m2, c->nn_Wb, c->nn_Wtb, c->nn_partial,
-                  c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal};
+                  c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal,
+                  c->nn_out};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t& e : c->nn_ev)
@@ -987,7 +1010,8 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
                hipMalloc(&c->nn_status, ncell) == hipSuccess &&
                hipMalloc(&c->nn_fbcount, sizeof(int32_t) * (ntile + nchunk)) == hipSuccess &&
                hipMalloc(&c->nn_fblist, sizeof(int32_t) * ncell) == hipSuccess &&
-               hipMalloc(&c->nn_fbtotal, sizeof(int32_t)) == hipSuccess;
+               hipMalloc(&c->nn_fbtotal, sizeof(int32_t)) == hipSuccess &&
+               hipMalloc(&c->nn_out, sizeof(float) * ncell * ns) == hipSuccess;
     if (!okm) {
       nn_free(c);
       set_error("f3d_nn_init: device allocation failed");
@@ -1140,15 +1164,23 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   const bool timed = c->nn_timing;
   if (timed) (void)hipEventRecord(c->nn_ev[0], s);
   // every tile writes its count, so no clearing is needed
+  const unsigned nb = nblocks(ncell);
   if (c->g.Ns == 9)
-    hipLaunchKernelGGL((k_nn_chem<9>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
+    hipLaunchKernelGGL((k_nn_net<9>), dim3((unsigned)ntile), dim3(64), 0, s, Q, rhoi, tag,
                        c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
-                       c->g, dt, c->nn_fbcount, st);
+                       c->g, c->nn_out, st);
   else
-    hipLaunchKernelGGL((k_nn_chem<5>), dim3((unsigned)ntile), dim3(64), 0, s, U, Q, rhoi, tag,
+    hipLaunchKernelGGL((k_nn_net<5>), dim3((unsigned)ntile), dim3(64), 0, s, Q, rhoi, tag,
                        c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
-                       c->g, dt, c->nn_fbcount, st);
+                       c->g, c->nn_out, st);
   if (timed) (void)hipEventRecord(c->nn_ev[1], s);
+  if (c->g.Ns == 9)
+    hipLaunchKernelGGL((k_nn_epi<9>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_out, c->nn_fbcount, st);
+  else
+    hipLaunchKernelGGL((k_nn_epi<5>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_out, c->nn_fbcount, st);
+  if (timed) (void)hipEventRecord(c->nn_ev[2], s);
   // tile counts -> tile offsets; the chunk sums sit after the tile counts
   const long nchunk = (ntile + kScanChunk - 1) / kScanChunk;
   int32_t* sums = c->nn_fbcount + ntile;
@@ -1161,7 +1193,7 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
                      ncell, c->nn_fblist);
   rc = hip_check(hipGetLastError(), "f3d_nn_chem");
   if (rc) return rc;
-  if (timed) (void)hipEventRecord(c->nn_ev[2], s);
+  if (timed) (void)hipEventRecord(c->nn_ev[3], s);
   int32_t nfb = 0;
   rc = hip_check(hipMemcpyAsync(&nfb, c->nn_fbtotal, sizeof(int32_t), hipMemcpyDeviceToHost, s),
                  "f3d_nn_chem");
@@ -1171,16 +1203,16 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   *n_fallback = nfb;
   // the rejected cells, in cell order: the stiff ones stay grouped in a few
   // waves (a wave's cost is its slowest cell) instead of one in every wave
-  if (timed) (void)hipEventRecord(c->nn_ev[3], s);  // after the read-back
+  if (timed) (void)hipEventRecord(c->nn_ev[4], s);  // after the read-back
   if (nfb > 0) {
     rc = launch_chem(c, 2, U, Q, rhoi, dt, maxiter, rtol, ttol, nullptr, tag, s, c->nn_fblist,
                      nfb);
     if (rc) return rc;
   }
   if (timed) {
-    // the fallback's end replaces the read-back mark: ev[3] - ev[2] then spans
+    // the fallback's end replaces the read-back mark: ev[4] - ev[3] then spans
     // the 4-byte read-back and the Newton fallback launch
-    (void)hipEventRecord(c->nn_ev[3], s);
+    (void)hipEventRecord(c->nn_ev[4], s);
     c->nn_timed = true;
   }
   return F3D_OK;
@@ -1203,10 +1235,10 @@ int f3d_nn_timing(f3d_ctx* c, int32_t enable, double* ms) {
       set_error("f3d_nn_timing: no timed f3d_nn_chem call yet");
       return F3D_ERR_STATE;
     }
-    int rc = hip_check(hipEventSynchronize(c->nn_ev[3]), "f3d_nn_timing");
+    int rc = hip_check(hipEventSynchronize(c->nn_ev[4]), "f3d_nn_timing");
     if (rc) return rc;
-    float t[3];
-    for (int k = 0; k < 3; ++k) {
+    float t[4];
+    for (int k = 0; k < 4; ++k) {
       rc = hip_check(hipEventElapsedTime(&t[k], c->nn_ev[k], c->nn_ev[k + 1]), "f3d_nn_timing");
       if (rc) return rc;
       ms[k] = t[k];

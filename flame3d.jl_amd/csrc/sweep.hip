@@ -161,7 +161,18 @@ __device__ inline bool sweep_line(const Geom& g, int dir, int seg, SweepLine& m)
   return true;
 }
 
+// The marches keep their loads two steps ahead of their use: the inputs of
+// the stencil row a step needs, its sensor value and the right-hand-side
+// entries it accumulates into are fetched two steps earlier into registers,
+// so each wave has three steps' loads in flight instead of waiting on each.
+constexpr int kAhead = 2;
+
 // wave 0 of a block: the five flow fluxes
+struct FlowIn {
+  double rho, u, v, w, p, e;
+  float a1, a2, a3, phi;
+};
+
 __device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __restrict__ Q,
                                        const float* __restrict__ A1, const float* __restrict__ A2,
                                        const float* __restrict__ A3,
@@ -171,25 +182,59 @@ __device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __
   const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
   const float sw = PT<float>::sweps2(P);
   float wp[5][8], wm[5][8];
-  auto load = [&](int r, int slot) {
+  // row r's inputs and the sensor of row r - 3 (the face f = r - 3 needs
+  // rows f - 4 .. f + 3 and the sensor of rows f - 1, f)
+  auto fetch = [&](int r, FlowIn& x, bool with_phi) {
     const long p = m.base + (long)(r + NG) * m.s;
+    x.rho = Q[p];
+    x.u = Q[p + NP];
+    x.v = Q[p + 2 * NP];
+    x.w = Q[p + 3 * NP];
+    x.p = Q[p + 4 * NP];
+    x.e = Q[p + 6 * NP];
+    x.a1 = A1[p];
+    x.a2 = A2[p];
+    x.a3 = A3[p];
+    x.phi = with_phi ? phi[p - 3 * m.s] : 0.0f;  // rows r >= c0 + 3 only: row r - 3 >= 0
+  };
+  auto split = [&](const FlowIn& x, int slot) {
     float fp[5], fm[5];
-    split_point<float>((float)Q[p], (float)Q[p + NP], (float)Q[p + 2 * NP], (float)Q[p + 3 * NP],
-                       (float)Q[p + 4 * NP], (float)Q[p + 6 * NP], A1[p], A2[p], A3[p], sw, fp,
-                       fm);
+    split_point<float>((float)x.rho, (float)x.u, (float)x.v, (float)x.w, (float)x.p, (float)x.e,
+                       x.a1, x.a2, x.a3, sw, fp, fm);
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       wp[n][slot] = fp[n];
       wm[n][slot] = fm[n];
     }
   };
+  auto fetch_rhs = [&](int f, float* o) {  // the entries of cell f - 1
+    const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) load(m.c0 - 4 + k, k);
+    for (int n = 0; n < 5; ++n) o[n] = (acc && f > m.c0 && f <= m.c1) ? rhs[n * NC + cell] : 0.0f;
+  };
+  {
+    FlowIn x;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      fetch(m.c0 - 4 + k, x, false);
+      split(x, k);
+    }
+  }
   float phA = phi[m.base + (long)(m.c0 - 1 + NG) * m.s];
+  const int last = m.c1 + 3;  // the last row the segment reads
+  FlowIn in[kAhead + 1];
+  float ro[kAhead + 1][5];
+#pragma unroll
+  for (int d = 0; d < kAhead; ++d) {
+    if (m.c0 + 3 + d <= last) fetch(m.c0 + 3 + d, in[d], true);
+    fetch_rhs(m.c0 + d, ro[d]);
+  }
   float prev[5];
   for (int f = m.c0; f <= m.c1; ++f) {
-    load(f + 3, 7);
-    const float phB = phi[m.base + (long)(f + NG) * m.s];
+    if (f + 3 + kAhead <= last) fetch(f + 3 + kAhead, in[kAhead], true);
+    fetch_rhs(f + kAhead, ro[kAhead]);
+    split(in[0], 7);
+    const float phB = in[0].phi;
     const int br = weno_branch<float>(fmaxf(phA, phB), P);
     float cur[5];
 #pragma unroll
@@ -197,7 +242,7 @@ __device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __
     if (f > m.c0) {
       const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
-      for (int n = 0; n < 5; ++n) rhs_put(rhs, n * NC + cell, prev[n] - cur[n], acc);
+      for (int n = 0; n < 5; ++n) rhs[n * NC + cell] = ro[0][n] + (prev[n] - cur[n]);
     }
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
@@ -208,11 +253,22 @@ __device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __
         wm[n][k] = wm[n][k + 1];
       }
     }
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      in[d] = in[d + 1];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) ro[d][n] = ro[d + 1][n];
+    }
     phA = phB;
   }
 }
 
 // waves 1..: species k0 .. k0+2
+struct SpecIn {
+  double u, v, w, r[3];
+  float a1, a2, a3, phi;
+};
+
 template <int NS>
 __device__ inline void conv_march_species(float* __restrict__ rhs, const double* __restrict__ Q,
                                           const double* __restrict__ rhoi,
@@ -224,26 +280,59 @@ __device__ inline void conv_march_species(float* __restrict__ rhs, const double*
                                           const SweepLine& m, int k0, int acc) {
   const long NP = g.NP, NC = (long)g.Nx * g.Ny * g.Nz;
   float wp[3][8], wm[3][8];
-  auto load = [&](int r, int slot) {
+  auto fetch = [&](int r, SpecIn& x, bool with_phi) {
     const long p = m.base + (long)(r + NG) * m.s;
-    const float un =
-        A1[p] * (float)Q[p + NP] + A2[p] * (float)Q[p + 2 * NP] + A3[p] * (float)Q[p + 3 * NP];
+    x.u = Q[p + NP];
+    x.v = Q[p + 2 * NP];
+    x.w = Q[p + 3 * NP];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) x.r[kk] = (k0 + kk < NS) ? rhoi[p + (long)(k0 + kk) * NP] : 0.0;
+    x.a1 = A1[p];
+    x.a2 = A2[p];
+    x.a3 = A3[p];
+    x.phi = with_phi ? phi[p - 3 * m.s] : 0.0f;  // rows r >= c0 + 3 only: row r - 3 >= 0
+  };
+  auto split = [&](const SpecIn& x, int slot) {
+    const float un = x.a1 * (float)x.u + x.a2 * (float)x.v + x.a3 * (float)x.w;
     const float Ep = 0.5f * (un + fabsf(un));
     const float Em = 0.5f * (un - fabsf(un));
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
-      const float r = (k0 + kk < NS) ? (float)rhoi[p + (long)(k0 + kk) * NP] : 0.0f;
+      const float r = (float)x.r[kk];
       wp[kk][slot] = Ep * r;
       wm[kk][slot] = Em * r;
     }
   };
+  auto fetch_rhs = [&](int f, float* o) {
+    const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) load(m.c0 - 4 + k, k);
+    for (int kk = 0; kk < 3; ++kk)
+      o[kk] = (acc && k0 + kk < NS && f > m.c0 && f <= m.c1) ? rhs[(5 + k0 + kk) * NC + cell]
+                                                             : 0.0f;
+  };
+  {
+    SpecIn x;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      fetch(m.c0 - 4 + k, x, false);
+      split(x, k);
+    }
+  }
   float phA = phi[m.base + (long)(m.c0 - 1 + NG) * m.s];
+  const int last = m.c1 + 3;
+  SpecIn in[kAhead + 1];
+  float ro[kAhead + 1][3];
+#pragma unroll
+  for (int d = 0; d < kAhead; ++d) {
+    if (m.c0 + 3 + d <= last) fetch(m.c0 + 3 + d, in[d], true);
+    fetch_rhs(m.c0 + d, ro[d]);
+  }
   float prev[3];
   for (int f = m.c0; f <= m.c1; ++f) {
-    load(f + 3, 7);
-    const float phB = phi[m.base + (long)(f + NG) * m.s];
+    if (f + 3 + kAhead <= last) fetch(f + 3 + kAhead, in[kAhead], true);
+    fetch_rhs(f + kAhead, ro[kAhead]);
+    split(in[0], 7);
+    const float phB = in[0].phi;
     const int br = weno_branch<float>(fmaxf(phA, phB), P);
     float cur[3];
 #pragma unroll
@@ -253,7 +342,7 @@ __device__ inline void conv_march_species(float* __restrict__ rhs, const double*
       const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
       for (int kk = 0; kk < 3; ++kk)
-        if (k0 + kk < NS) rhs_put(rhs, (5 + k0 + kk) * NC + cell, prev[kk] - cur[kk], acc);
+        if (k0 + kk < NS) rhs[(5 + k0 + kk) * NC + cell] = ro[0][kk] + (prev[kk] - cur[kk]);
     }
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk) {
@@ -263,6 +352,12 @@ __device__ inline void conv_march_species(float* __restrict__ rhs, const double*
         wp[kk][k] = wp[kk][k + 1];
         wm[kk][k] = wm[kk][k + 1];
       }
+    }
+#pragma unroll
+    for (int d = 0; d < kAhead; ++d) {
+      in[d] = in[d + 1];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) ro[d][kk] = ro[d + 1][kk];
     }
     phA = phB;
   }

@@ -299,9 +299,10 @@ int f3d_nn_chem(f3d_ctx* ctx, double* U, double* Q, double* rhoi, double dt, int
 
 /* Timing of f3d_nn_chem's parts with HIP events recorded on the call's stream
  * (no reference counterpart: a measurement hook).  enable = 1 records events
- * in later calls, 0 stops; ms (host, may be NULL) receives the last timed
- * call's [network kernel, tile scan + compaction, read-back + Newton
- * fallback] in milliseconds (synchronizes on that call's end). */
+ * in later calls, 0 stops; ms (host, may be NULL, 4 doubles) receives the
+ * last timed call's [network kernel, gate epilogue kernel, tile scan +
+ * compaction, read-back + Newton fallback] in milliseconds (synchronizes on
+ * that call's end). */
 int f3d_nn_timing(f3d_ctx* ctx, int32_t enable, double* ms);
 
 /* ---- stage level -------------------------------------------------------- */
