@@ -18,8 +18,8 @@ constexpr int kNewtonMaxSub = 64;
 // One backward-Euler solve over h of the reactor (rho_k, T) from (r, T),
 // total energy fixed at E0: Newton on
 //   F_k = rho_k - rs_k - h W_k wdot_k(rho, T) = 0,  F_E = E(rho, T) - E0 = 0
-// with the exact mass-action Jacobian (+ third-body terms) and a forward
-// difference in T.  Returns false when the iterate leaves the admissible set
+// with the exact mass-action Jacobian (+ third-body terms) and the analytic
+// temperature derivative of the rates.  Returns false when the iterate leaves the admissible set
 // (non-finite, T outside [T_lo, T_hi], rho_k < -rho_tot) or maxiter
 // iterations pass without convergence; `it` counts the iterations taken.
 template <int NS>
@@ -37,10 +37,9 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
     for (int n = 0; n < NS; ++n) sc[n] = r[n] / P->mw[n] * 1e-6;
     // the rate Jacobian is built in the top-left block of the system matrix
     // and turned into I - W_n/W_l h dwdot/dc in place (one matrix in registers)
-    double A[N1][N1], wdot2[NS], rhs[N1], dx[N1];
-    production_rate<NS, true, true, N1>(P, sc, T, A, wdot);
-    const double hT = 1e-6 * T;
-    production_rate<NS, false, false, N1>(P, sc, T + hT, A, wdot2);  // A untouched (JAC=false)
+    double A[N1][N1], rhs[N1], dx[N1];
+    // rate Jacobian in A[0..NS-1][0..NS-1], d wdot / dT in A[0..NS-1][NS]
+    production_rate<NS, true, true, N1, true>(P, sc, T, A, wdot);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const double src = wdot[n] * P->mw[n] * 1e6 * h;
@@ -48,7 +47,7 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
 #pragma unroll
       for (int l = 0; l < NS; ++l)
         A[n][l] = (n == l ? 1.0 : 0.0) - A[n][l] * P->mw[n] / P->mw[l] * h;
-      A[n][NS] = -((wdot2[n] - wdot[n]) / hT) * P->mw[n] * 1e6 * h;
+      A[n][NS] = -A[n][NS] * P->mw[n] * 1e6 * h;
     }
     double E = internal_energy<NS>(P, T, r);
 #pragma unroll

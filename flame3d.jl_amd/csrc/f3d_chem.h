@@ -106,13 +106,32 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 // falloff rates (Lindemann part).
 // A is any matrix with at least NS x NS entries (the Newton solve passes its
 // (NS+1) x (NS+1) system so the rate Jacobian is built in place).
-template <int NS, bool JAC, bool FULL = false, int NA = NS>
+// TDER (Newton mode, NA = NS + 1): the last column A[l][NS] receives
+// d wdot_l / dT at fixed concentrations, analytically: d ln kf/dT =
+// (b + (E/R)/T)/T, d ln Kc/dT = sgm/T - sum_m nu_m (h_m/RT)/T (Gibbs-Helmholtz
+// on the NASA-7 fits), and for a Troe falloff the derivatives of Pr/(1 + Pr)
+// and of F through Fcent(T) and Pr(T).
+template <int NS, bool JAC, bool FULL = false, int NA = NS, bool TDER = false>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
                                        double (&A)[NA][NA], double* wdot) {
   const double lgT = f3d_log(T);
   const double invT = 1.0 / T;
   double gi[NS];
   gibbs<NS>(P, T, lgT, invT, gi);
+  double hRT[TDER ? NS : 1];  // h/RT of each species (NASA-7), for d ln Kc / dT
+  if (TDER) {
+    const double T1 = 0.5 * T;
+    const double T2 = 1.0 / 3.0 * T * T;
+    const double T3 = 0.25 * T * T * T;
+    const double T4 = 0.2 * T * T * T * T;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const double* c = (T < P->sep[n]) ? P->lo[n] : P->hi[n];
+      hRT[n] = c[0] + c[1] * T1 + c[2] * T2 + c[3] * T3 + c[4] * T4 + c[5] * invT;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) A[l][NA - 1] = 0.0;
+  }
   const double RsT = P->Ru / P->atm * 1e6 * T;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -140,6 +159,7 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
     const int rt = P->rtype[n];
     double dlnq_dM = 0.0;
     double fac = 1.0;  // third-body / falloff factor multiplying both directions
+    double dlnfac = 0.0;  // d ln fac / dT (TDER)
     if (rt == 2) {
       double mix = 0.0;
 #pragma unroll
@@ -155,9 +175,11 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       const double redP = mix / kf * P->loP[n][0] * f3d_exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
       const double logPred = f3d_log10(redP);
       const double A_ = P->Troe[n][0];
-      const double logFcent = f3d_log10((1 - A_) * f3d_exp(-T / P->Troe[n][1]) +
-                                        A_ * f3d_exp(-T / P->Troe[n][2]) +
-                                        f3d_exp(-P->Troe[n][3] * invT));
+      const double e3 = f3d_exp(-T / P->Troe[n][1]);
+      const double e1 = f3d_exp(-T / P->Troe[n][2]);
+      const double e2 = f3d_exp(-P->Troe[n][3] * invT);
+      const double Fcent = (1 - A_) * e3 + A_ * e1 + e2;
+      const double logFcent = f3d_log10(Fcent);
       const double troe_c = -0.4 - 0.67 * logFcent;
       const double troe_n = 0.75 - 1.27 * logFcent;
       const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
@@ -167,6 +189,24 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       q_r *= k_1;
       dlnq_dM = 1.0 / ((1.0 + redP) * mix);
       fac = k_1;
+      if (TDER) {
+        const double ln10 = 2.30258509299404568402;
+        // d ln Pr / dT: k0 and kinf Arrhenius
+        const double dlnPr = (P->loP[n][1] - P->Arr[n][1]) * invT +
+                             (P->loP[n][2] - P->Arr[n][2]) * invT * invT;
+        const double dL = dlnPr / ln10;
+        const double dFc = -(1 - A_) / P->Troe[n][1] * e3 - A_ / P->Troe[n][2] * e1 +
+                           P->Troe[n][3] * invT * invT * e2;
+        const double dLc = dFc / (Fcent * ln10);
+        const double u = troe_c + logPred;
+        const double v = troe_n - 0.14 * u;
+        const double du = -0.67 * dLc + dL;
+        const double dv = -1.27 * dLc - 0.14 * du;
+        const double dtroe = (du * v - u * dv) / (v * v);
+        const double w = 1.0 + troe * troe;
+        const double dlog10F = dLc / w - logFcent * (2.0 * troe * dtroe) / (w * w);
+        dlnfac = dlnPr / (1.0 + redP) + ln10 * dlog10F;
+      }
     }
     if (JAC && FULL && rt != 1) {
       const double dqM = (q_f - q_r) * dlnq_dM;
@@ -211,6 +251,19 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       }
     }
     const double dq = q_f - q_r;
+    if (TDER) {
+      double dgs = 0.0;  // sum_m (nu_f - nu_r)_m d(g_m/RT)/dT = -sum_m nu_m h_m/RT / T
+#pragma unroll
+      for (int m = 0; m < NS; ++m) dgs += hRT[m] * (double)(P->vf[n][m] - P->vr[n][m]);
+      const double dlnkf = P->Arr[n][1] * invT + P->Arr[n][2] * invT * invT;
+      const double dlnKc = (double)P->sgm[n] * invT - dgs * invT;
+      const double dqT = q_f * (dlnkf + dlnfac) - q_r * (dlnkf - dlnKc + dlnfac);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int dl = P->vr[n][l] - P->vf[n][l];
+        if (dl != 0) A[l][NA - 1] += dqT * (double)dl;
+      }
+    }
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
       const int vf = P->vf[n][m], vr = P->vr[n][m];

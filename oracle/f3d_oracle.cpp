@@ -250,9 +250,13 @@ inline CR xlog10(CR a) { return log10(a); }
 inline CR xexp10(CR a) { return pow(CR(10.0), a); }
 
 // vproductionRate / vproductionRate_Jac, reactions.jl:311-460 (see or_production_rate)
+// dwdT (may be null): d wdot / dT at fixed concentrations, analytically
+// (d ln kf/dT = (b + (E/R)/T)/T; d ln Kc/dT = sgm/T - sum_m nu_m (h_m/RT)/T by
+// Gibbs-Helmholtz on the NASA-7 fits; a Troe falloff adds the derivatives of
+// Pr/(1 + Pr) and of F through Fcent(T) and Pr(T)), in the library's order.
 template <class D>
 void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D* sc, D T,
-             double jac_eps, int32_t full_jac, D* wdot, D* Arate) {
+             double jac_eps, int32_t full_jac, D* wdot, D* Arate, D* dwdT = nullptr) {
   const int Ns = g.Ns, Nr = g.Nr;
   std::vector<D> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
   for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
@@ -263,6 +267,18 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
   for (int n = 0; n < Nr; ++n)
     kf[n] = kin->Arr[3 * n] * xexp(kin->Arr[3 * n + 1] * lgT - kin->Arr[3 * n + 2] * invT);
   gibbs_t<D>(g, th, T, lgT, invT, gi.data());
+  std::vector<D> hRT(dwdT ? Ns : 0);
+  if (dwdT) {
+    const D T1 = 0.5 * T;
+    const D T2 = 1.0 / 3.0 * T * T;
+    const D T3 = 0.25 * T * T * T;
+    const D T4 = 0.2 * T * T * T * T;
+    for (int m = 0; m < Ns; ++m) {
+      const double* c = (dval(T) < th->coeffs_sep[m]) ? th->coeffs_lo + 7 * m : th->coeffs_hi + 7 * m;
+      hRT[m] = c[0] + c[1] * T1 + c[2] * T2 + c[3] * T3 + c[4] * T4 + c[5] * invT;
+    }
+    for (int l = 0; l < Ns; ++l) dwdT[l] = 0.0;
+  }
   D RsT = th->Ru / kin->atm * 1e6 * T;
   for (int n = 0; n < Nr; ++n) {
     D dg = 0;
@@ -280,6 +296,7 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
     }
     D mixture = 0.0;
     D dlnq_dM = 0.0;  // d ln(q) / d[M], used by the full Jacobian only
+    D dlnfac = 0.0;   // d ln(third-body / falloff factor) / dT
     kfac_den[n] = 0.0;
     if (kin->reaction_type[n] == 2) {
       for (int m = 0; m < Ns; ++m) mixture += kin->ef[m + Ns * n] * sc[m];
@@ -294,8 +311,11 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
       D redP = mixture / kf[n] * lp[0] * xexp(lp[1] * lgT - lp[2] * invT);
       D logPred = xlog10(redP);
       double A = tr[0];
-      D logFcent =
-          xlog10((1 - A) * xexp(-T / tr[1]) + A * xexp(-T / tr[2]) + xexp(-tr[3] * invT));
+      D e3 = xexp(-T / tr[1]);
+      D e1 = xexp(-T / tr[2]);
+      D e2 = xexp(-tr[3] * invT);
+      D Fcent = (1 - A) * e3 + A * e1 + e2;
+      D logFcent = xlog10(Fcent);
       D troe_c = -0.4 - 0.67 * logFcent;
       D troe_n = 0.75 - 1.27 * logFcent;
       D troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
@@ -305,6 +325,21 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
       q_r *= k_1;
       dlnq_dM = 1.0 / ((1.0 + redP) * mixture);
       kfac_den[n] = k_1;
+      if (dwdT) {
+        const double ln10 = 2.30258509299404568402;
+        D dlnPr = (lp[1] - kin->Arr[3 * n + 1]) * invT + (lp[2] - kin->Arr[3 * n + 2]) * invT * invT;
+        D dL = dlnPr / ln10;
+        D dFc = -(1 - A) / tr[1] * e3 - A / tr[2] * e1 + tr[3] * invT * invT * e2;
+        D dLc = dFc / (Fcent * ln10);
+        D u = troe_c + logPred;
+        D v = troe_n - 0.14 * u;
+        D du = -0.67 * dLc + dL;
+        D dv = -1.27 * dLc - 0.14 * du;
+        D dtroe = (du * v - u * dv) / (v * v);
+        D w = 1.0 + troe * troe;
+        D dlog10F = dLc / w - logFcent * (2.0 * troe * dtroe) / (w * w);
+        dlnfac = dlnPr / (1.0 + redP) + ln10 * dlog10F;
+      }
     }
     if (Arate && full_jac && dlnq_dM != 0.0) {
       // third-body / falloff dependence through [M] = sum ef_m c_m
@@ -344,6 +379,18 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
           Arate[l + Ns * m] += dd * (double)(kin->vr[l + Ns * n] - kin->vf[l + Ns * n]);
         (void)kfac;
         (void)krac;
+      }
+    }
+    if (dwdT) {
+      D dgs = 0.0;
+      for (int m = 0; m < Ns; ++m)
+        dgs += hRT[m] * (double)(kin->vf[m + Ns * n] - kin->vr[m + Ns * n]);
+      D dlnkf = kin->Arr[3 * n + 1] * invT + kin->Arr[3 * n + 2] * invT * invT;
+      D dlnKc = (double)kin->sgm[n] * invT - dgs * invT;
+      D dqT = q_f * (dlnkf + dlnfac) - q_r * (dlnkf - dlnKc + dlnfac);
+      for (int l = 0; l < Ns; ++l) {
+        long dl = kin->vr[l + Ns * n] - kin->vf[l + Ns * n];
+        if (dl != 0) dwdT[l] += dqT * (double)dl;
       }
     }
     for (int m = 0; m < Ns; ++m) {
@@ -404,15 +451,13 @@ bool be_solve_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, D* r
   D dT_prev = 0.0;
   for (int k = 0; k < maxiter; ++k) {
     for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
-    prate_t<D>(g, th, kin, s, T, kEps, 1, wdot, Arate.data());
-    D hT = 1e-6 * T;
-    prate_t<D>(g, th, kin, s, T + hT, kEps, 1, wdot2, static_cast<D*>(nullptr));
+    prate_t<D>(g, th, kin, s, T, kEps, 1, wdot, Arate.data(), wdot2);  // wdot2: d wdot / dT
     for (int n = 0; n < Ns; ++n) {
       D src = wdot[n] * th->mw[n] * 1e6 * h;
       rhs[n] = -(r[n] - rs[n] - src);
       for (int l = 0; l < Ns; ++l)
         A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * h;
-      A[n + N1 * Ns] = -((wdot2[n] - wdot[n]) / hT) * th->mw[n] * 1e6 * h;
+      A[n + N1 * Ns] = -wdot2[n] * th->mw[n] * 1e6 * h;
     }
     D E = internal_energy_t<D>(g, th, T, r);
     for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
@@ -740,6 +785,13 @@ void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics
                         const double* sc, double T, double jac_eps, int32_t full_jac,
                         double* wdot, double* Arate) {
   prate_t<double>(Grid(d), th, kin, sc, T, jac_eps, full_jac, wdot, Arate);
+}
+
+// the rates with their analytic temperature derivative at fixed
+// concentrations (the backward-Euler Newton's last Jacobian column)
+void or_production_rate_dT(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                           const double* sc, double T, double* wdot, double* dwdT) {
+  prate_t<double>(Grid(d), th, kin, sc, T, kEps, 1, wdot, static_cast<double*>(nullptr), dwdT);
 }
 
 // solve, reactions.jl:463-503: Gaussian elimination with partial pivoting,

@@ -61,6 +61,8 @@ void or_gibbs(const or_dims* d, const or_thermo* th, double T, double* g);
 void or_production_rate(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
                         const double* sc, double T, double jac_eps, int32_t full_jac,
                         double* wdot, double* Arate /* [Ns,Ns] or NULL */);
+void or_production_rate_dT(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
+                           const double* sc, double T, double* wdot, double* dwdT);
 void or_gauss_solve(int32_t n, double* a /* [n,n] col-major, destroyed */,
                     double* b /* destroyed */, double* x);
 void or_set_diffusivity_reference_form(int32_t on);

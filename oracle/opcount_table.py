@@ -7,7 +7,7 @@ writes profiles/chem_opcount.json, which the benchmark reads as data.
 
 Per mechanism (9-species H2/O2, 5-species air):
   * newton_iteration: one iteration of the backward-Euler Newton solve
-    (rates with the exact Jacobian, the rates at T + dT for dR/dT, the
+    (rates with the exact Jacobian and the analytic dR/dT, the
     (Ns+1)^2 system, its pivoted solve, the update), mean over reactors of
     the 128^3 chemistry workload's distribution (T ~ U[900, 2500] K,
     P ~ U[1, 10] atm, Beta(2,2) mixture fraction; seed 12345) for H2/O2 and

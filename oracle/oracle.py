@@ -173,6 +173,15 @@ class Oracle:
                                     _f(A) if jac else None)
         return (w, A) if jac else w
 
+    def production_rate_dT(self, sc, T):
+        """(wdot, d wdot / dT at fixed concentrations) from the analytic derivative."""
+        sc = np.ascontiguousarray(sc, dtype=np.float64)
+        w = np.zeros(self.Ns)
+        dw = np.zeros(self.Ns)
+        self.lib.or_production_rate_dT(C.byref(self.dims), C.byref(self.thermo), C.byref(self.kin),
+                                       _f(sc), C.c_double(T), _f(w), _f(dw))
+        return w, dw
+
     def gauss_solve(self, A, b):
         A = np.asfortranarray(A, dtype=np.float64).copy(order="F")
         b = np.array(b, dtype=np.float64)

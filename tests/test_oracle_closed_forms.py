@@ -626,3 +626,36 @@ def test_chemistry_exp_log_accuracy(h2):
     assert math.isnan(L.or_exp(float("nan"))) and math.isnan(L.or_log(-1.0))
     assert L.or_exp(800.0) == math.inf and L.or_exp(-800.0) == 0.0
     assert L.or_log(0.0) == -math.inf and L.or_log(math.inf) == math.inf
+
+
+@pytest.mark.parametrize("mname", ["h2_lidryer", "air5"])
+def test_rates_temperature_derivative(mname, air, h2):
+    """The analytic d wdot / dT of the backward-Euler Newton's Jacobian
+    (Arrhenius, Gibbs-Helmholtz for K_c, the Troe falloff's Pr/(1+Pr) and F)
+    equals the central difference of the rates (step 1e-5 T; Richardson
+    extrapolation of two steps for the O(h^4) check), species by species,
+    over states on both sides of the NASA-7 breakpoint.  A dropped Troe term
+    or a wrong sign in d ln K_c / dT fails it."""
+    from oracle import Oracle
+    m = {"air5": air, "h2_lidryer": h2}[mname]
+    orc = Oracle(1, 1, 1, m)
+    rng = np.random.default_rng(12)
+    temps = (700.0, 1200.0, 1800.0, 2600.0) if mname == "h2_lidryer" else (2500.0, 4500.0)
+    for T in temps:
+        for _ in range(3):
+            Y = rng.uniform(0.01, 1.0, m.Ns)
+            Y /= Y.sum()
+            P = rng.uniform(1e4, 1e6)
+            W = 1.0 / np.sum(Y / m.mw)
+            rho = P * W / (m.Ru * T)
+            sc = Y * rho / m.mw * 1e-6
+            w, dw = orc.production_rate_dT(sc, T)
+            fd = []
+            for h in (1e-5 * T, 0.5e-5 * T):
+                wp = orc.production_rate(sc, T + h, jac=False)
+                wm = orc.production_rate(sc, T - h, jac=False)
+                fd.append((wp - wm) / (2 * h))
+            rich = (4 * fd[1] - fd[0]) / 3
+            scale = np.max(np.abs(rich)) + 1e-300
+            err = np.max(np.abs(dw - rich)) / scale
+            assert err < 1e-6, (mname, T, err)
