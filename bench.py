@@ -371,19 +371,35 @@ def run_path(prec, case, mech, steps, warmup, torch, abi, kernel_reps, surrogate
         kt = time_kernels(ctx, torch, Q, rhoi, case.grid.metrics, case.grid.J, kernel_reps)
     extra = {}
     if chemistry and surrogate is not None:
-        # the surrogate step's parts (network kernel, scan + compaction,
-        # read-back + Newton fallback), event-timed inside the library on the
-        # launch stream, over two further calls from copies of the end state
-        Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
+        # the surrogate step's parts (network kernel, gate epilogue, scan +
+        # compaction, read-back + Newton fallback), event-timed inside the
+        # library on the launch stream, over two further calls, each from a
+        # copy of the end state
         ctx.nn_timing(True)
         parts, nfb = [], []
         for _ in range(2):
+            Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
             nfb.append(ctx.nn_chem(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6))
             parts.append(ctx.nn_timing())
         ctx.nn_timing(False)
         extra["nn_parts_ms"] = [sum(p[k] for p in parts) / len(parts) for k in range(4)]
         extra["nn_parts_fallback"] = sum(nfb) / len(nfb)
-        del Uc, Qc, rc
+        # the fallback's Newton iterations: the Newton step over every cell
+        # from the same state counts them per cell (the fallback runs the same
+        # kernel on the rejected cells, so their counts are the same)
+        N3 = (N[2], N[1], N[0])
+        st = torch.zeros(N3, dtype=torch.int8, device="cuda")
+        iters = torch.zeros(N3, dtype=torch.int32, device="cuda")
+        Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
+        ctx.nn_chem(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6, status=st)
+        Uc, Qc, rc = U.clone(), Q.clone(), rhoi.clone()
+        ctx.chem_newton(Uc, Qc, rc, case.dt / 2, 50, 1e-10, 1e-6, iters=iters)
+        torch.cuda.synchronize()
+        rej = st >= 2
+        extra["fallback_iterations"] = int(iters.abs()[rej].to(torch.int64).sum())
+        extra["fallback_cells"] = int(rej.sum())
+        extra["nn_status_counts"] = {str(c): int((st == c).sum()) for c in range(9)}
+        del Uc, Qc, rc, st, iters
     elif chemistry:
         # one more Newton call from copies of the end state with the
         # iteration counts returned (the FP64 work of the launch)
@@ -862,6 +878,11 @@ def main():
     nn_f64 = hot * ops["surrogate_epilogue"]["flop"]
     nn_bf16 = hot * MLP_FLOP_PER_CELL
     chem_tab = {
+        "k_nn_net": {"ms": net_ms, "launches_per_step": 2, "cells": hot,
+                     "bf16_mfma_flop": nn_bf16,
+                     "bf16_mfma_frac": nn_bf16 / (net_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS},
+        "k_nn_epi": {"ms": epi_ms, "launches_per_step": 2, "cells": hot, "fp64_flop": nn_f64,
+                     "fp64_frac": nn_f64 / (epi_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "k_nn_chem": {"ms": nn_ms, "launches_per_step": 2, "cells": hot,
                       "fp64_flop": nn_f64, "fp64_TFLOPs": nn_f64 / (nn_ms * 1e-3) / 1e12,
                       "bf16_mfma_flop": nn_bf16,
@@ -869,13 +890,20 @@ def main():
                       "bf16_mfma_frac": nn_bf16 / (nn_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS},
         "nn_scan_compact": {"ms": nn_scan_ms, "launches_per_step": 2},
         "newton_fallback": {"ms": nn_fb_ms, "launches_per_step": 2,
-                            "cells": r32["nn_parts_fallback"]},
+                            "cells": r32["fallback_cells"],
+                            "iterations": r32["fallback_iterations"],
+                            "fp64_flop": r32["fallback_iterations"] * ops["newton_iteration"]["flop"]
+                            + r32["fallback_cells"] * ops["newton_cell_overhead"]["flop"],
+                            "gate_codes": r32["nn_status_counts"]},
         "k_chem_newton_fp64_plain": {
             "ms": r64["newton_ms"], "launches_per_step": 2, "cells": r64["newton_cells"],
             "iterations": r64["newton_iterations"],
             "fp64_flop": r64["newton_iterations"] * ops["newton_iteration"]["flop"]
             + r64["newton_cells"] * ops["newton_cell_overhead"]["flop"]},
     }
+    kf = chem_tab["newton_fallback"]
+    kf["fp64_TFLOPs"] = kf["fp64_flop"] / (kf["ms"] * 1e-3) / 1e12
+    kf["fp64_frac"] = kf["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
     kn = chem_tab["k_chem_newton_fp64_plain"]
     kn["fp64_TFLOPs"] = kn["fp64_flop"] / (kn["ms"] * 1e-3) / 1e12
     kn["fp64_frac"] = kn["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
@@ -893,9 +921,11 @@ def main():
                     "alg_fp64_flop_per_launch": nn_f64, "avg_launch_ms": nn_ms,
                     "bf16_mfma_frac": kc["bf16_mfma_frac"]}
     elif dom == "newton_fallback":
-        roofline = {"bound": "mfma", "kernel": "k_chem Newton fallback", "achieved": None,
-                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
-                    "traffic": None, "avg_launch_ms": nn_fb_ms}
+        roofline = {"bound": "mfma", "kernel": "k_chem<9, 2> (backward-Euler Newton on the "
+                    "surrogate's rejected cells; FP64 vector, against the FP64 peak)", "achieved": kf["fp64_TFLOPs"],
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": kf["fp64_frac"],
+                    "traffic": None, "alg_fp64_flop_per_launch": kf["fp64_flop"],
+                    "avg_launch_ms": nn_fb_ms}
     else:
         dk = ktab["fp32"][dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS,
