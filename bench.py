@@ -809,7 +809,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU's block only; --gpus {a.gpus} is not "
-              "supported (the multi-GPU decomposition is design-only, see DESIGN.md)",
+              "supported (the block decomposition over several GPUs is a design, not built)",
               file=sys.stderr)
         sys.exit(2)
 
