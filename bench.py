@@ -109,12 +109,57 @@ def _pmc_avg(path):
     return {k: sum(v) / len(v) for k, v in agg.items()}
 
 
+def _pmc_multi(path):
+    """{kernel: {counter: mean per dispatch}} of a multi-counter run."""
+    import csv
+    import re
+    agg = {}
+    for r in csv.DictReader(open(path)):
+        short = re.sub(r"\(.*", "", r["Kernel_Name"]).strip()
+        agg.setdefault(short, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def valu_summary(d):
+    """Per-kernel instruction counters per dispatch from the runs pmc_sq1 and
+    pmc_sq2 under `d` (SQ counters, summed over the GPU by rocprofv3; one
+    count per wave instruction): FP64 flop = 64 (ADD + MUL + 2 FMA) (the
+    transcendental helpers TRANS_F64 listed apart), BF16 MFMA flop =
+    512 MOPS, the share of the waves' lifetime spent issuing VALU
+    instructions (ACTIVE_INST_VALU / WAVE_CYCLES) and waiting
+    (WAIT_INST_ANY / WAVE_CYCLES)."""
+    runs = {}
+    for sub in ("pmc_sq1", "pmc_sq2"):
+        path = os.path.join(d, sub, "pmc_counter_collection.csv")
+        if os.path.exists(path):
+            for k, cs in _pmc_multi(path).items():
+                runs.setdefault(k, {}).update(cs)
+    out = {}
+    for k, c in runs.items():
+        if "f3d::" not in k:
+            continue
+        e = dict(c)
+        if all(x in c for x in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64")):
+            e["fp64_flop"] = 64.0 * (c["SQ_INSTS_VALU_ADD_F64"] + c["SQ_INSTS_VALU_MUL_F64"] +
+                                     2.0 * c["SQ_INSTS_VALU_FMA_F64"])
+        if "SQ_INSTS_VALU_MFMA_MOPS_BF16" in c:
+            e["bf16_mfma_flop"] = 512.0 * c["SQ_INSTS_VALU_MFMA_MOPS_BF16"]
+        if c.get("SQ_WAVE_CYCLES"):
+            if "SQ_ACTIVE_INST_VALU" in c:
+                e["valu_active_per_wave_cycle"] = c["SQ_ACTIVE_INST_VALU"] / c["SQ_WAVE_CYCLES"]
+            if "SQ_WAIT_INST_ANY" in c:
+                e["wait_per_wave_cycle"] = c["SQ_WAIT_INST_ANY"] / c["SQ_WAVE_CYCLES"]
+        out[k] = e
+    return out
+
+
 def pmc_summary(d):
     """Per-kernel HBM bytes per dispatch from the counter CSVs of four runs
     under `d` (pmc_fetch, pmc_write: the benchmark; copy_fetch, copy_write:
     --calibrate-copy, a 1 GiB device copy).  FETCH_SIZE / WRITE_SIZE are in
     KiB per dispatch; each is scaled by (known copy bytes / counter reading
-    of the copy kernel)."""
+    of the copy kernel).  With the SQ-counter runs present (valu_summary),
+    their per-kernel instruction counts are added under "valu"."""
     f = _pmc_avg(os.path.join(d, "pmc_fetch", "pmc_counter_collection.csv"))
     w = _pmc_avg(os.path.join(d, "pmc_write", "pmc_counter_collection.csv"))
     cf = _pmc_avg(os.path.join(d, "copy_fetch", "pmc_counter_collection.csv"))
@@ -130,6 +175,9 @@ def pmc_summary(d):
             continue
         out["kernels"][k] = {"fetch_bytes": f[k] * 1024.0 * kf,
                              "write_bytes": w.get(k, 0.0) * 1024.0 * kw}
+    v = valu_summary(d)
+    if v:
+        out["valu"] = v
     return out
 
 
@@ -482,7 +530,8 @@ def mark_flips(torch, m, ca, cb):
     return n
 
 
-def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), radius=2):
+def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), radius=2,
+                           scheme=None, attach=None, legs=None):
     """Untimed runs of `steps` time steps from the same initial state, each
     compared after every step with FP64 plain (FP64 stage + FP64 Newton
     chemistry) by the interior's max |dT| and max_k |dY_k|:
@@ -499,13 +548,16 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), rad
     of a face flipped at any stage so far (T_linf_away_K).
     Returns per-step curves, the values at `marks` and at the last step, and
     the first step at which each path leaves the bar (in Linf, and away from
-    the flips)."""
+    the flips).  scheme: the contexts' scheme options; attach: loads the
+    surrogate into a context (default: the H2/O2 network); legs: the names
+    of the comparisons to run (default: all four)."""
+    attach = attach or (lambda c: sur.attach(c))
     N = case.grid.Nx, case.grid.Ny, case.grid.Nz
     sl = (slice(4, -4),) * 3
     keep_all = steps <= 40
 
     def run(prec, nn, sink, perturb=0.0):
-        ctx = abi.Context(*N, mech, prec)
+        ctx = abi.Context(*N, mech, prec, scheme=scheme)
         ctx.set_metrics(case.grid.metrics, case.grid.J)
         ctx.set_boundary(case.bc)
         Q, rhoi = case.Q.clone(), case.rhoi.clone()
@@ -518,7 +570,7 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), rad
         ctx.prim2c(U, Q)
         Un, rn = ctx.zeros(5), ctx.zeros(mech.Ns)
         if nn:
-            sur.attach(ctx)
+            attach(ctx)
         phi1, phi2 = ctx.scheme["hybrid_phi1"], ctx.scheme["hybrid_phi2"]
 
         def half():
@@ -554,6 +606,8 @@ def accuracy_decomposition(case, mech, steps, torch, abi, sur, marks=(1, 6), rad
     for name, prec, nn, pert in (("fp32_newton", "fp32", False, 0.0), ("fp64_nn", "fp64", True, 0.0),
                                  ("fp32_nn", "fp32", True, 0.0),
                                  ("fp64_perturbed", "fp64", False, 1e-7)):
+        if legs is not None and name not in legs:
+            continue
         curve = {"step": [], "T_linf_K": [], "Y_linf": [], "cells_T_over_1K": [],
                  "face_flips": [], "cells_near_flips": [], "T_linf_away_K": [],
                  "Y_linf_away": []}
@@ -742,12 +796,14 @@ def bench_jet(a, torch, abi, cases, mm):
                         scheme=scheme)
     c64.close()
     ms64 = r64["ms_total"] / stages
-    log(f"[bench] FP64 + Newton {ms64:.3f} ms/stage (chemistry {r64['chem_ms_mean']:.2f} ms/call)")
+    chem64 = r64["chem_ms_mean"]
+    log(f"[bench] FP64 + Newton {ms64:.3f} ms/stage (chemistry {chem64:.2f} ms/call)")
     c32, r32 = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
                         scheme=scheme)
     c32.close()
     ms32 = r32["ms_total"] / stages
-    log(f"[bench] FP32 + Newton {ms32:.3f} ms/stage (chemistry {r32['chem_ms_mean']:.2f} ms/call)")
+    chem32 = r32["chem_ms_mean"]
+    log(f"[bench] FP32 + Newton {ms32:.3f} ms/stage (chemistry {chem32:.2f} ms/call)")
     cnn, rnn = run_path("fp32", case, mech, a.steps, a.warmup, torch, abi, a.kernel_reps,
                         scheme=scheme, surrogate=lambda c: sur.attach(c, sur.AIR_PATH))
     cnn.close()
@@ -759,6 +815,17 @@ def bench_jet(a, torch, abi, cases, mm):
     names = ["rho", "u", "v", "w", "p", "T", "e"]
     npts = (N[0] + 8) * (N[1] + 8) * (N[2] + 8)
     nf = (N[0] + 1) * N[1] * N[2]
+    err_nn = state_errors(rnn, r64, mech)
+    err_32 = state_errors(r32, r64, mech)
+    fields = field_errors(torch, rnn["Q"], r64["Q"], names)
+    del r64, r32
+    accuracy = None
+    if not a.no_accuracy:
+        # the same steps untimed, every stage's hybrid-switch branches compared
+        accuracy = accuracy_decomposition(
+            case, mech, a.steps, torch, abi, sur, marks=(1, 10, 25), scheme=scheme,
+            attach=lambda c: sur.attach(c, sur.AIR_PATH),
+            legs=("fp32_newton", "fp32_nn", "fp64_perturbed"))
     ktab = {}
     for name, (ms, per) in rnn["kernels"].items():
         b = kernel_bytes(name, 4, mech.Ns, npts, nf, nc)
@@ -773,15 +840,14 @@ def bench_jet(a, torch, abi, cases, mm):
         "config": {"workload": f"hot air jet {N[0]}x{N[1]}x{N[2]}, 5-species air, 5000 K Ma 1.5 "
                                "into 300 K, WENO7-Z/UP7/NND, Strang chemistry", "dt_s": case.dt},
         "fp64_newton": {"value": nc / (ms64 * 1e-3), "ms_per_stage": ms64,
-                        "chem_ms_per_call": r64["chem_ms_mean"]},
+                        "chem_ms_per_call": chem64},
         "fp32_newton": {"value": nc / (ms32 * 1e-3), "ms_per_stage": ms32,
-                        "chem_ms_per_call": r32["chem_ms_mean"],
-                        "error_vs_fp64": state_errors(r32, r64, mech)},
+                        "chem_ms_per_call": chem32, "error_vs_fp64": err_32},
         "fp32_nn": {"ms_per_stage": msnn, "chem_ms_per_call": rnn["chem_ms_mean"],
                     "fallback_fraction": fb},
         "speedup_vs_fp64_plain": ms64 / msnn, "speedup_fp32_newton_vs_fp64": ms64 / ms32,
-        "error_vs_fp64_plain": state_errors(rnn, r64, mech),
-        "error_fields_vs_fp64_plain": field_errors(torch, rnn["Q"], r64["Q"], names),
+        "error_vs_fp64_plain": err_nn, "error_fields_vs_fp64_plain": fields,
+        "accuracy": accuracy,
         "roofline": stage_roofline(ktab, mech.Ns), "kernels": {"fp32": ktab}}))
 
 

@@ -99,6 +99,13 @@ static int fill_params(DevParams& P, const f3d_dims* d, const f3d_thermo* th,
       P.c8_f[n][l] = (float)(1.0 / std::sqrt(8.0 + 8.0 / wr));
       P.iwr_f[n][l] = (float)(1.0 / wr);
     }
+  for (int n = 0; n < d->Ns; ++n) {
+    P.rmw[n] = 1.0 / P.mw[n];
+    P.Rmw[n] = P.Ru / P.mw[n];
+    P.cmw[n] = 1e-6 / P.mw[n];
+    P.hfm[n] = P.lo[n][5] * P.Ru / P.mw[n];
+    for (int l = 0; l < d->Ns; ++l) P.wnl[n][l] = P.mw[n] / P.mw[l];
+  }
   P.Ru_f = (float)P.Ru;
   P.WENOeps_f = (float)P.WENOeps;
   P.phi1_f = (float)P.phi1;

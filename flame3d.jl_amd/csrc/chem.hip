@@ -19,7 +19,9 @@ constexpr int kNewtonMaxSub = 64;
 // total energy fixed at E0: Newton on
 //   F_k = rho_k - rs_k - h W_k wdot_k(rho, T) = 0,  F_E = E(rho, T) - E0 = 0
 // with the exact mass-action Jacobian (+ third-body terms) and the analytic
-// temperature derivative of the rates.  Returns false when the iterate leaves the admissible set
+// temperature derivative of the rates.  Constant quotients (1e-6/W_n,
+// W_n/W_l, Ru/W_n, 1/W_n, the reverse rates' 1/RsT, the pivots' reciprocals)
+// are multiplied where the reference's point-implicit step divides.  Returns false when the iterate leaves the admissible set
 // (non-finite, T outside [T_lo, T_hi], rho_k < -rho_tot) or maxiter
 // iterations pass without convergence; `it` counts the iterations taken.
 template <int NS>
@@ -34,22 +36,21 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
   double dT_prev = 0.0;
   for (int k = 0; k < maxiter; ++k) {
 #pragma unroll
-    for (int n = 0; n < NS; ++n) sc[n] = r[n] / P->mw[n] * 1e-6;
+    for (int n = 0; n < NS; ++n) sc[n] = r[n] * P->cmw[n];
     // the rate Jacobian is built in the top-left block of the system matrix
     // and turned into I - W_n/W_l h dwdot/dc in place (one matrix in registers)
     double A[N1][N1], rhs[N1], dx[N1];
     // rate Jacobian in A[0..NS-1][0..NS-1], d wdot / dT in A[0..NS-1][NS]
-    production_rate<NS, true, true, N1, true>(P, sc, T, A, wdot);
+    production_rate<NS, true, true, N1, true, true>(P, sc, T, A, wdot);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const double src = wdot[n] * P->mw[n] * 1e6 * h;
       rhs[n] = -(r[n] - rs[n] - src);
 #pragma unroll
-      for (int l = 0; l < NS; ++l)
-        A[n][l] = (n == l ? 1.0 : 0.0) - A[n][l] * P->mw[n] / P->mw[l] * h;
+      for (int l = 0; l < NS; ++l) A[n][l] = (n == l ? 1.0 : 0.0) - A[n][l] * P->wnl[n][l] * h;
       A[n][NS] = -A[n][NS] * P->mw[n] * 1e6 * h;
     }
-    double E = internal_energy<NS>(P, T, r);
+    double E = internal_energy<NS, true>(P, T, r);
 #pragma unroll
     for (int n = 0; n < NS; ++n) E += hf[n] * r[n];
     rhs[NS] = -(E - E0);
@@ -68,11 +69,11 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
           e = (lo[0] - 1) * T + lo[1] * T2 + lo[2] * T3 + lo[3] * T4 + lo[4] * T5;
         else
           e = (hc[0] - 1) * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 + (hc[5] - lo[5]);
-        A[NS][l] = e * P->Ru / P->mw[l] + hf[l];
+        A[NS][l] = e * P->Rmw[l] + hf[l];
       }
     }
-    A[NS][NS] = cv_mix<NS>(P, T, r);
-    gauss_solve<N1>(A, rhs, dx);
+    A[NS][NS] = cv_mix<NS, true>(P, T, r);
+    gauss_solve<N1, true>(A, rhs, dx);
     double rho = 0;
 #pragma unroll
     for (int n = 0; n < NS; ++n) rho += r[n];
@@ -161,10 +162,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       r0[n] = r[n];
-      hf[n] = P->lo[n][5] * P->Ru / P->mw[n];
+      hf[n] = P->hfm[n];
       rho_tot += r0[n];
     }
-    double E0 = internal_energy<NS>(P, T0, r0);
+    double E0 = internal_energy<NS, true>(P, T0, r0);
 #pragma unroll
     for (int n = 0; n < NS; ++n) E0 += hf[n] * r0[n];
     const double T_lo = 0.5 * P->min_temp, T_hi = 1.5 * P->max_temp;

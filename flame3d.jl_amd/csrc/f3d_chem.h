@@ -111,7 +111,9 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 // (b + (E/R)/T)/T, d ln Kc/dT = sgm/T - sum_m nu_m (h_m/RT)/T (Gibbs-Helmholtz
 // on the NASA-7 fits), and for a Troe falloff the derivatives of Pr/(1 + Pr)
 // and of F through Fcent(T) and Pr(T).
-template <int NS, bool JAC, bool FULL = false, int NA = NS, bool TDER = false>
+// RCP (Newton mode): the reverse rate constant as kf (RsT)^-sgm exp(-dg)
+// with 1/RsT formed once, instead of kf / Kc per reaction.
+template <int NS, bool JAC, bool FULL = false, int NA = NS, bool TDER = false, bool RCP = false>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
                                        double (&A)[NA][NA], double* wdot) {
   const double lgT = f3d_log(T);
@@ -133,6 +135,7 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
     for (int l = 0; l < NS; ++l) A[l][NA - 1] = 0.0;
   }
   const double RsT = P->Ru / P->atm * 1e6 * T;
+  const double iRsT = RCP ? 1.0 / RsT : 0.0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
     wdot[m] = 0.0;
@@ -147,9 +150,15 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
     double dg = 0;
 #pragma unroll
     for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-    const double Kc = ipow(RsT, P->sgm[n]) * f3d_exp(dg);
+    double kr;  // reverse rate constant kf / Kc, Kc = (RsT)^sgm exp(dg)
+    if (RCP) {
+      const int sg = P->sgm[n];
+      kr = kf * ((sg > 0 ? ipow(iRsT, sg) : ipow(RsT, -sg)) * f3d_exp(-dg));
+    } else {
+      kr = kf / (ipow(RsT, P->sgm[n]) * f3d_exp(dg));
+    }
     double q_f = kf;
-    double q_r = q_f / Kc;
+    double q_r = kr;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
       const int vf = P->vf[n][m], vr = P->vr[n][m];
@@ -236,7 +245,7 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
           dqf = pr;
         }
         if (vrm != 0) {
-          double pr = kf / Kc * (double)vrm * ipow(sc[m], vrm - 1);
+          double pr = kr * (double)vrm * ipow(sc[m], vrm - 1);
 #pragma unroll
           for (int j = 0; j < NS; ++j)
             if (j != m && P->vr[n][j] != 0) pr *= ipow(sc[j], P->vr[n][j]);
@@ -297,6 +306,7 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
   double gi[NS], cc[3];
   gibbs<NS>(P, T, lgT, invT, gi);
   const double RsT = P->Ru / P->atm * 1e6 * T;
+  const double iRsT = 1.0 / RsT;
 #pragma unroll
   for (int m = 0; m < NS; ++m) wdot[m] = 0.0;
 #pragma unroll
@@ -314,8 +324,8 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     double dg = 0;
 #pragma unroll
     for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-    const double Kc = ipow(RsT, P->sgm[n]) * f3d_exp(dg);
-    const double kr = kf / Kc;
+    const int sg = P->sgm[n];  // kr = kf / Kc, Kc = (RsT)^sgm exp(dg), with 1/RsT formed once
+    const double kr = kf * ((sg > 0 ? ipow(iRsT, sg) : ipow(RsT, -sg)) * f3d_exp(-dg));
     // q_f, q_r: the full mass-action products (same order as production_rate,
     // so wdot is bit-identical to it); o_f, o_r: the products over the
     // species that are not chain carriers, shared by the three derivatives
@@ -409,10 +419,12 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
 }
 
 // solve, reactions.jl:463-503 (partial pivoting; row exchanges done with
-// selects so the matrix stays in registers).  Only the columns >= k that
-// the back substitution reads are eliminated; they are bit-identical to the
-// reference's full-row update.
-template <int N>
+// selects so the matrix stays in registers, skipped for a column where no
+// lane of the wave exchanges rows).  Only the columns >= k that the back
+// substitution reads are eliminated; they are bit-identical to the
+// reference's full-row update.  RCP (Newton mode): each row's factor is
+// -a[i][k] times 1/a[k][k], formed once per column, instead of divided.
+template <int N, bool RCP = false>
 __device__ inline void gauss_solve(double (&a)[N][N], double (&b)[N], double (&x)[N]) {
 #pragma unroll
   for (int k = 0; k < N - 1; ++k) {
@@ -426,22 +438,25 @@ __device__ inline void gauss_solve(double (&a)[N][N], double (&b)[N], double (&x
         kk = i;
       }
     }
+    if (__any(kk != k)) {
 #pragma unroll
-    for (int i = k + 1; i < N; ++i) {
-      const bool sw = (kk == i);
+      for (int i = k + 1; i < N; ++i) {
+        const bool sw = (kk == i);
 #pragma unroll
-      for (int j = k; j < N; ++j) {
-        const double t = a[k][j];
-        a[k][j] = sw ? a[i][j] : t;
-        a[i][j] = sw ? t : a[i][j];
+        for (int j = k; j < N; ++j) {
+          const double t = a[k][j];
+          a[k][j] = sw ? a[i][j] : t;
+          a[i][j] = sw ? t : a[i][j];
+        }
+        const double t = b[k];
+        b[k] = sw ? b[i] : t;
+        b[i] = sw ? t : b[i];
       }
-      const double t = b[k];
-      b[k] = sw ? b[i] : t;
-      b[i] = sw ? t : b[i];
     }
+    const double inv = RCP ? 1.0 / a[k][k] : 0.0;
 #pragma unroll
     for (int i = k + 1; i < N; ++i) {
-      const double f = -a[i][k] / a[k][k];
+      const double f = RCP ? -a[i][k] * inv : -a[i][k] / a[k][k];
 #pragma unroll
       for (int j = k + 1; j < N; ++j) a[i][j] += f * a[k][j];
       b[i] += f * b[k];

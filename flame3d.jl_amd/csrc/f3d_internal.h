@@ -29,6 +29,13 @@ struct DevParams {
   int vf[MAXNR][MAXNS], vr[MAXNR][MAXNS];
   double Arr[MAXNR][3], ef[MAXNR][MAXNS], loP[MAXNR][3], Troe[MAXNR][4];
   double SWeps2, WENOeps, phi1, phi2, UP7[7], T_criteria;
+  // Quotients of constants, rounded once on the host as the device would
+  // round them: 1/W_n, Ru/W_n, 1e-6/W_n, W_n/W_l, and the formation energy
+  // per mass lo[n][5] Ru / W_n in the reference's operation order.  The
+  // backward-Euler Newton, the warm-started e -> T and the surrogate's
+  // epilogue multiply by them where the reference divides (a correctly
+  // rounded FP64 division is a ten-instruction sequence on the GPU).
+  double rmw[MAXNS], Rmw[MAXNS], cmw[MAXNS], hfm[MAXNS], wnl[MAXNS][MAXNS];
   int recon_kind;  // 0 UP7 / WENO5-Z / NND (the reference), 1 WENO7-Z in the middle branch
   // float copies for the FP32 flux path
   float mw_f[MAXNS], sep_f[MAXNS], lo_f[MAXNS][7], hi_f[MAXNS][7];
@@ -184,8 +191,10 @@ __device__ inline void h_specs(const DevParams* P, R T, R* h) {
   }
 }
 
-// ei_specs, thermo.jl:136-160 and InternalEnergy, thermo.jl:219-228 (FP64)
-template <int NS>
+// ei_specs, thermo.jl:136-160 and InternalEnergy, thermo.jl:219-228 (FP64).
+// RCP: rho_k e_k times 1/W_k instead of divided by W_k (one rounding
+// differs; the Newton chemistry and the warm-started e -> T use it).
+template <int NS, bool RCP = false>
 __device__ inline double internal_energy(const DevParams* P, double T, const double* rhoi) {
   const double T2 = 0.5 * T * T;
   const double T3 = 2.0 / 3.0 * T2 * T;
@@ -202,13 +211,13 @@ __device__ inline double internal_energy(const DevParams* P, double T, const dou
     } else {
       e = (hc[0] - 1) * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 + (hc[5] - lo[5]);
     }
-    result += rhoi[n] * e / P->mw[n];
+    result += RCP ? rhoi[n] * e * P->rmw[n] : rhoi[n] * e / P->mw[n];
   }
   return result * P->Ru;
 }
 
-// CV, thermo.jl:194-203 (FP64)
-template <int NS>
+// CV, thermo.jl:194-203 (FP64); RCP as internal_energy
+template <int NS, bool RCP = false>
 __device__ inline double cv_mix(const DevParams* P, double T, const double* rhoi) {
   const double T2 = T * T;
   const double T3 = T2 * T;
@@ -218,17 +227,17 @@ __device__ inline double cv_mix(const DevParams* P, double T, const double* rhoi
   for (int n = 0; n < NS; ++n) {
     const double* c = (T < P->sep[n]) ? P->lo[n] : P->hi[n];
     const double cp = c[0] + c[1] * T + c[2] * T2 + c[3] * T3 + c[4] * T4;
-    result += (cp - 1) * rhoi[n] / P->mw[n];
+    result += RCP ? (cp - 1) * rhoi[n] * P->rmw[n] : (cp - 1) * rhoi[n] / P->mw[n];
   }
   return result * P->Ru;
 }
 
-// Pmixture, thermo.jl:20-26
-template <int NS>
+// Pmixture, thermo.jl:20-26; RCP as internal_energy
+template <int NS, bool RCP = false>
 __device__ inline double pmixture(const DevParams* P, double T, const double* rhoi) {
   double YOW = 0;
 #pragma unroll
-  for (int n = 0; n < NS; ++n) YOW += rhoi[n] / P->mw[n];
+  for (int n = 0; n < NS; ++n) YOW += RCP ? rhoi[n] * P->rmw[n] : rhoi[n] / P->mw[n];
   return P->Ru * T * YOW;
 }
 
@@ -249,6 +258,7 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
   double t1 = ein / As;
   if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
   int it = 0;
+  double x1 = 0.0, x2 = 0.0;  // t1 one and two updates ago
   for (int m = 0; m < 30; ++m) {
     const double e1 = internal_energy<NS>(P, t1, rhoi);
     const double cv = cv_mix<NS>(P, t1, rhoi);
@@ -262,6 +272,17 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
     } else if (t1 + dt == t1) {
       break;
     }
+    // Each update is a function of t1 alone, so once t1 equals the iterate
+    // two updates back exactly (an energy inside the jump of the NASA-7
+    // fits at a breakpoint, where the updates alternate across it) the rest
+    // cycles with period two and the point the 30 updates end on is known:
+    // the same result, bit for bit, without the remaining updates.
+    if (m >= 2 && t1 == x2) {
+      if (iters) *iters = 30;
+      return ((30 - m) % 2 == 0) ? t1 : x1;
+    }
+    x2 = x1;
+    x1 = t1;
     t1 += dt;
     ++it;
   }
@@ -324,7 +345,8 @@ __device__ inline double get_T_gap(const DevParams* P, double ein, const double*
 // An update that flips sign without shrinking marks an energy inside the
 // small jump of the NASA-7 fits at their breakpoint (e.g. a stream at
 // 1000 K), which has no exact root: there, and for any result within 0.5 K
-// of a breakpoint, the result is get_T's own end point (get_T_gap).
+// of a breakpoint, the result is get_T's own end point (get_T_gap).  The
+// warm updates evaluate the energy and c_v with the reciprocal molar masses.
 template <int NS>
 __device__ inline double get_T_from(const DevParams* P, double ein, const double* rhoi,
                                     double T_guess) {
@@ -335,8 +357,8 @@ __device__ inline double get_T_from(const DevParams* P, double ein, const double
   bool done = false;
   double dt_prev = 0.0;
   for (int m = 0; m < 30; ++m) {
-    const double e1 = internal_energy<NS>(P, t1, rhoi);
-    const double cv = cv_mix<NS>(P, t1, rhoi);
+    const double e1 = internal_energy<NS, true>(P, t1, rhoi);
+    const double cv = cv_mix<NS, true>(P, t1, rhoi);
     double dt = (ein - e1) / cv;
     if (dt > 100.0) {
       dt = 100.0;

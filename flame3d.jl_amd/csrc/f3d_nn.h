@@ -71,6 +71,7 @@ struct NNDev {
   float elem[4][NN_OUT];           // atoms of element e per molecule of species k
   double dt;                       // chemistry time step the network was trained for
   double proj[NN_OUT][NN_OUT];     // element-conserving projection of dY (identity if none)
+  double elem_mw[4][NN_OUT];       // elem[e][k] / W_k (atoms of e per kg of species k, / 1000)
 };
 
 }  // namespace f3d

@@ -375,7 +375,7 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
   for (int k = 0; k < NS; ++k) {
     double e[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) e[a] = a < ne ? (double)C->elem[a][k] / P->mw[k] : 0.0;
+    for (int a = 0; a < 4; ++a) e[a] = a < ne ? C->elem_mw[a][k] : 0.0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       b[a] += e[a] * dY0[k];
@@ -429,7 +429,7 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
   for (int k = 0; k < NS; ++k) {
     double v = 0.0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) v += (a < ne ? (double)C->elem[a][k] / P->mw[k] : 0.0) * z[a];
+    for (int a = 0; a < 4; ++a) v += (a < ne ? C->elem_mw[a][k] : 0.0) * z[a];
     dY[k] = dY0[k] - S[k] * v;
   }
   return ok;
@@ -580,7 +580,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
         double num = 0.0;
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
-          const double ae = (double)C->elem[e][n] / P->mw[n];
+          const double ae = C->elem_mw[e][n];
           num += ae * dY0[n];
           gross += ae * fabs(dY0[n]);
         }
@@ -610,15 +610,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
     if (code == 1 && C->dT_max > 0.0f) {
       double esum = 0.0;
 #pragma unroll
-      for (int n = 0; n < NS; ++n) esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * dY[n]);
-      if (!(esum * rho <= (double)C->dT_max * cv_mix<NS>(P, T, r))) code = 5;
+      for (int n = 0; n < NS; ++n) esum += fabs(P->hfm[n] * dY[n]);
+      if (!(esum * rho <= (double)C->dT_max * cv_mix<NS, true>(P, T, r))) code = 5;
     }
     if (code == 1) {
       double rn[NS], dei = 0.0, sr = 0.0;
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
         rn[n] = fmax(Y[n] + dY[n], 0.0) * rho;
-        dei += -(P->lo[n][5] * P->Ru / P->mw[n]) * (rn[n] - r[n]);
+        dei += -P->hfm[n] * (rn[n] - r[n]);
       }
       // the state chem_epilogue would write, evaluated before committing
       const double rhoc = fmax(rho, kEps);
@@ -634,21 +634,21 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
       if (res_on || C->chi_max > 0.0f || C->chain_max > 0.0f) {
         double sc[NS], wdot[NS], Adum[NS][NS], Mc[3][3];
 #pragma unroll
-        for (int n = 0; n < NS; ++n) sc[n] = rn[n] / P->mw[n] * 1e-6;
+        for (int n = 0; n < NS; ++n) sc[n] = rn[n] * P->cmw[n];
         if (C->chain_max > 0.0f) {
           const int cidx[3] = {C->chain_idx[0], C->chain_idx[1], C->chain_idx[2]};
           production_rate_block3<NS>(P, sc, Tn, cidx, wdot, Mc);
         } else {
           production_rate<NS, false>(P, sc, Tn, Adum, wdot);
         }
-        const double cvn = cv_mix<NS>(P, Tn, rn);
+        const double cvn = cv_mix<NS, true>(P, Tn, rn);
         if (res_on) {
           double rmax = 0.0, esum = 0.0;
 #pragma unroll
           for (int n = 0; n < NS; ++n) {
             const double R = rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt;
             rmax = fmax(rmax, fabs(R));
-            esum += fabs(P->lo[n][5] * P->Ru / P->mw[n] * R);
+            esum += fabs(P->hfm[n] * R);
           }
           if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
           if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cvn)) code = 6;
@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
       }
       if (code == 1) {
         U[p + 4 * NP] += dei;
-        Q[p + 4 * NP] = fmax(pmixture<NS>(P, Tn, rn), kEps);
+        Q[p + 4 * NP] = fmax(pmixture<NS, true>(P, Tn, rn), kEps);
         Q[p + 5 * NP] = Tn;
         Q[p + 6 * NP] = ein;
 #pragma unroll
@@ -992,7 +992,10 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
   h.y_floor = cfg->y_floor;
   h.n_elem = cfg->n_elem;
   for (int e = 0; e < 4; ++e)
-    for (int k = 0; k < NN_OUT; ++k) h.elem[e][k] = cfg->elem[e][k];
+    for (int k = 0; k < NN_OUT; ++k) {
+      h.elem[e][k] = cfg->elem[e][k];
+      h.elem_mw[e][k] = k < c->g.Ns ? (double)h.elem[e][k] / c->host_params.mw[k] : 0.0;
+    }
   h.dt = cfg->dt;
   if (nn_projection(c, cfg, h.proj)) {
     set_error("f3d_nn_init: element matrix is rank deficient (an element with no species?)");

@@ -162,23 +162,28 @@ void gibbs_t(const Grid& g, const or_thermo* th, D T, D lgT, D invT, D* gi) {
   }
 }
 
-// CV, thermo.jl:194-203 (J/m^3/K)
-template <class D>
+// CV, thermo.jl:194-203 (J/m^3/K).  RCP: each term times 1/W_n (the
+// quotient rounded once) instead of divided by W_n, as the library's Newton
+// chemistry and warm-started GetT evaluate it (one rounding differs from the
+// reference's order).
+template <class D, bool RCP = false>
 D cv_t(const Grid& g, const or_thermo* th, D T, const D* rhoi) {
   D cp[64];
   cp_specs_t(g, th, T, cp);
   D result = 0;
-  for (int n = 0; n < g.Ns; ++n) result += (cp[n] - 1) * rhoi[n] / th->mw[n];
+  for (int n = 0; n < g.Ns; ++n)
+    result += RCP ? (cp[n] - 1) * rhoi[n] * (1.0 / th->mw[n]) : (cp[n] - 1) * rhoi[n] / th->mw[n];
   return result * th->Ru;
 }
 
-// InternalEnergy, thermo.jl:219-228 (J/m^3)
-template <class D>
+// InternalEnergy, thermo.jl:219-228 (J/m^3); RCP as cv_t
+template <class D, bool RCP = false>
 D internal_energy_t(const Grid& g, const or_thermo* th, D T, const D* rhoi) {
   D ei[64];
   ei_specs_t(g, th, T, ei);
   D result = 0;
-  for (int n = 0; n < g.Ns; ++n) result += rhoi[n] * ei[n] / th->mw[n];
+  for (int n = 0; n < g.Ns; ++n)
+    result += RCP ? rhoi[n] * ei[n] * (1.0 / th->mw[n]) : rhoi[n] * ei[n] / th->mw[n];
   return result * th->Ru;
 }
 
@@ -254,11 +259,14 @@ inline CR xexp10(CR a) { return pow(CR(10.0), a); }
 // (d ln kf/dT = (b + (E/R)/T)/T; d ln Kc/dT = sgm/T - sum_m nu_m (h_m/RT)/T by
 // Gibbs-Helmholtz on the NASA-7 fits; a Troe falloff adds the derivatives of
 // Pr/(1 + Pr) and of F through Fcent(T) and Pr(T)), in the library's order.
+// rcp (the library's Newton mode): the reverse rate constant as
+// kf (RsT)^-sgm exp(-dg) with 1/RsT formed once, instead of kf / Kc.
 template <class D>
 void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D* sc, D T,
-             double jac_eps, int32_t full_jac, D* wdot, D* Arate, D* dwdT = nullptr) {
+             double jac_eps, int32_t full_jac, D* wdot, D* Arate, D* dwdT = nullptr,
+             bool rcp = false) {
   const int Ns = g.Ns, Nr = g.Nr;
-  std::vector<D> gi(Ns), kf(Nr), Kc(Nr), kfac_den(Nr);
+  std::vector<D> gi(Ns), kf(Nr), kr(Nr), kfac_den(Nr);
   for (int m = 0; m < Ns; ++m) wdot[m] = 0.0;
   if (Arate)
     for (int m = 0; m < Ns * Ns; ++m) Arate[m] = 0.0;
@@ -280,15 +288,20 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
     for (int l = 0; l < Ns; ++l) dwdT[l] = 0.0;
   }
   D RsT = th->Ru / kin->atm * 1e6 * T;
+  D iRsT = rcp ? D(1.0) / RsT : D(0.0);
   for (int n = 0; n < Nr; ++n) {
     D dg = 0;
     for (int m = 0; m < Ns; ++m)
       dg += gi[m] * (double)(kin->vf[m + Ns * n] - kin->vr[m + Ns * n]);
-    Kc[n] = ipow_t(RsT, kin->sgm[n]) * xexp(dg);
+    const long sg = kin->sgm[n];
+    if (rcp)
+      kr[n] = kf[n] * ((sg > 0 ? ipow_t(iRsT, sg) : ipow_t(RsT, -sg)) * xexp(-dg));
+    else
+      kr[n] = kf[n] / (ipow_t(RsT, sg) * xexp(dg));  // kf / Kc
   }
   for (int n = 0; n < Nr; ++n) {
     D q_f = kf[n];
-    D q_r = q_f / Kc[n];
+    D q_r = kr[n];
     for (int m = 0; m < Ns; ++m) {
       long vf = kin->vf[m + Ns * n], vr = kin->vr[m + Ns * n];
       if (vf != 0) q_f *= ipow_t(sc[m], vf);
@@ -367,7 +380,7 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
           dqf = pr;
         }
         if (vrm != 0) {
-          D pr = kf[n] / Kc[n] * (double)vrm * ipow_t(sc[m], vrm - 1);
+          D pr = kr[n] * (double)vrm * ipow_t(sc[m], vrm - 1);
           for (int j = 0; j < Ns; ++j)
             if (j != m && kin->vr[j + Ns * n] != 0) pr *= ipow_t(sc[j], kin->vr[j + Ns * n]);
           dqr = pr;
@@ -406,9 +419,11 @@ void prate_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, const D
   }
 }
 
-// solve, reactions.jl:463-503 (see or_gauss_solve)
+// solve, reactions.jl:463-503 (see or_gauss_solve).  rcp (the library's
+// Newton mode): each row's factor is -A(i,k) times 1/A(k,k), formed once per
+// column, instead of divided by A(k,k).
 template <class D>
-void gauss_t(int32_t n, D* a, D* b, D* x) {
+void gauss_t(int32_t n, D* a, D* b, D* x, bool rcp = false) {
   using std::fabs;
   auto A = [&](int i, int j) -> D& { return a[i + n * j]; };
   for (int k = 0; k < n - 1; ++k) {
@@ -423,8 +438,9 @@ void gauss_t(int32_t n, D* a, D* b, D* x) {
     }
     for (int j = k; j < n; ++j) std::swap(A(k, j), A(kk, j));
     std::swap(b[k], b[kk]);
+    const D inv = rcp ? D(1.0) / A(k, k) : D(0.0);
     for (int i = k + 1; i < n; ++i) {
-      D f = -A(i, k) / A(k, k);
+      D f = rcp ? -A(i, k) * inv : -A(i, k) / A(k, k);
       for (int j = 0; j < n; ++j) A(i, j) += f * A(k, j);
       b[i] += f * b[k];
     }
@@ -450,22 +466,24 @@ bool be_solve_t(const Grid& g, const or_thermo* th, const or_kinetics* kin, D* r
   for (int n = 0; n < Ns; ++n) rs[n] = r[n];
   D dT_prev = 0.0;
   for (int k = 0; k < maxiter; ++k) {
-    for (int n = 0; n < Ns; ++n) s[n] = r[n] / th->mw[n] * 1e-6;
-    prate_t<D>(g, th, kin, s, T, kEps, 1, wdot, Arate.data(), wdot2);  // wdot2: d wdot / dT
+    // the constant quotients 1e-6/W_n, W_n/W_l, Ru/W_n and 1/W_n multiply
+    // (each rounded once, as the library precomputes them)
+    for (int n = 0; n < Ns; ++n) s[n] = r[n] * (1e-6 / th->mw[n]);
+    prate_t<D>(g, th, kin, s, T, kEps, 1, wdot, Arate.data(), wdot2, true);  // wdot2: d wdot / dT
     for (int n = 0; n < Ns; ++n) {
       D src = wdot[n] * th->mw[n] * 1e6 * h;
       rhs[n] = -(r[n] - rs[n] - src);
       for (int l = 0; l < Ns; ++l)
-        A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * th->mw[n] / th->mw[l] * h;
+        A[n + N1 * l] = (n == l ? 1.0 : 0.0) - Arate[n + Ns * l] * (th->mw[n] / th->mw[l]) * h;
       A[n + N1 * Ns] = -wdot2[n] * th->mw[n] * 1e6 * h;
     }
-    D E = internal_energy_t<D>(g, th, T, r);
+    D E = internal_energy_t<D, true>(g, th, T, r);
     for (int n = 0; n < Ns; ++n) E += hf[n] * r[n];
     rhs[Ns] = -(E - E0);
     ei_specs_t<D>(g, th, T, eis);
-    for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * th->Ru / th->mw[l] + hf[l];
-    A[Ns + N1 * Ns] = cv_t<D>(g, th, T, r);
-    gauss_t<D>(N1, A.data(), rhs, dx);
+    for (int l = 0; l < Ns; ++l) A[Ns + N1 * l] = eis[l] * (th->Ru / th->mw[l]) + hf[l];
+    A[Ns + N1 * Ns] = cv_t<D, true>(g, th, T, r);
+    gauss_t<D>(N1, A.data(), rhs, dx, true);
     D rho = 0;
     for (int n = 0; n < Ns; ++n) rho += r[n];
     double mx = 0;
@@ -648,7 +666,7 @@ double or_get_T_gap(const or_dims* d, const or_thermo* th, double ein, const dou
 
 // GetT warm-started from T_guess (the library's get_T_from, used on the FP32
 // stage's c2Prim and by the surrogate step): the Newton update of GetT from
-// T_guess; a guess or result outside [tmin, tmax] takes or_get_T, an energy
+// T_guess (energy and c_v with 1/W_n multiplied, as cv_t<RCP>); a guess or result outside [tmin, tmax] takes or_get_T, an energy
 // inside the NASA-7 breakpoint jump (an update that flips sign without
 // shrinking) or a result within 0.5 K of a breakpoint or_get_T_gap.  Parity unpinned against the reference (it has no
 // warm start); pinned here by agreement with or_get_T to the 1e-3 K
@@ -660,9 +678,10 @@ double or_get_T_from(const or_dims* d, const or_thermo* th, double ein, const do
   double t1 = T_guess;
   bool done = false;
   double dt_prev = 0.0;
+  Grid g(d);
   for (int m = 0; m < maxiter; ++m) {
-    const double e1 = or_internal_energy(d, th, t1, rhoi);
-    const double cv = or_cv(d, th, t1, rhoi);
+    const double e1 = internal_energy_t<double, true>(g, th, t1, rhoi);
+    const double cv = cv_t<double, true>(g, th, t1, rhoi);
     double dt = (ein - e1) / cv;
     if (dt > 100.0) {
       dt = 100.0;
@@ -1614,7 +1633,7 @@ void or_chem_newton(const or_dims* d, double* U, double* Q, double* rhoi, double
           hf[n] = th->coeffs_lo[5 + 7 * n] * th->Ru / th->mw[n];
           rho_tot += r0[n];
         }
-        double E0 = or_internal_energy(d, th, T0, r0);
+        double E0 = internal_energy_t<double, true>(g, th, T0, r0);
         for (int n = 0; n < Ns; ++n) E0 += hf[n] * r0[n];
         // one step of dt; on failure restart from the initial state with
         // twice as many backward-Euler substeps (at most kNewtonMaxSub)
@@ -1731,7 +1750,7 @@ void or_newton_opcount(const or_dims* d, const or_thermo* th, const or_kinetics*
     rho_tot += rhoi[n];
     r[n] = rhoi[n];
   }
-  double E0 = or_internal_energy(d, th, T0, rhoi);
+  double E0 = internal_energy_t<double, true>(g, th, T0, rhoi);
   for (int n = 0; n < Ns; ++n) E0 += hf[n] * rhoi[n];
   CR T = T0;
   int it = 0;
@@ -1741,30 +1760,31 @@ void or_newton_opcount(const or_dims* d, const or_thermo* th, const or_kinetics*
   tally_out(out, it);
 }
 
-// One production-rate evaluation (reactions.jl:311-460): full_jac = 0 rates
-// only, 1 rates and the exact mass-action Jacobian.
+// One production-rate evaluation (reactions.jl:311-460) as the library's
+// Newton mode and surrogate gates run it (reverse rates by the reciprocal
+// form): full_jac = 0 rates only, 1 rates and the exact mass-action Jacobian.
 void or_rates_opcount(const or_dims* d, const or_thermo* th, const or_kinetics* kin,
                       const double* rhoi, double T, int32_t full_jac, double* out) {
   Grid g(d);
   const int Ns = g.Ns;
   CR s[64], wdot[64];
   std::vector<CR> A(Ns * Ns);
-  for (int n = 0; n < Ns; ++n) s[n] = rhoi[n] / th->mw[n] * 1e-6;
+  for (int n = 0; n < Ns; ++n) s[n] = rhoi[n] * (1e-6 / th->mw[n]);
   g_tally = OpTally();
-  prate_t<CR>(g, th, kin, s, CR(T), kEps, 1, wdot, full_jac ? A.data() : nullptr);
+  prate_t<CR>(g, th, kin, s, CR(T), kEps, 1, wdot, full_jac ? A.data() : nullptr, nullptr, true);
   tally_out(out, 1);
 }
 
-// One e -> T Newton update (GetT, thermo.jl:231-282): InternalEnergy and CV
-// at T, the update (ein - e)/cv.
+// One warm-started e -> T Newton update (GetT's update, thermo.jl:231-282, with
+// the reciprocal molar masses): InternalEnergy and CV at T, (ein - e)/cv.
 void or_getT_update_opcount(const or_dims* d, const or_thermo* th, const double* rhoi, double T,
                             double ein, double* out) {
   Grid g(d);
   CR r[64];
   for (int n = 0; n < g.Ns; ++n) r[n] = rhoi[n];
   g_tally = OpTally();
-  CR e1 = internal_energy_t<CR>(g, th, CR(T), r);
-  CR cv = cv_t<CR>(g, th, CR(T), r);
+  CR e1 = internal_energy_t<CR, true>(g, th, CR(T), r);
+  CR cv = cv_t<CR, true>(g, th, CR(T), r);
   CR dT = (CR(ein) - e1) / cv;
   (void)dT;
   tally_out(out, 1);
