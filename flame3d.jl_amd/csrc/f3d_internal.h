@@ -298,7 +298,10 @@ __device__ inline double get_T(const DevParams* P, double ein, const double* rho
 // is get_T's iteration (same start, update, clamp and stopping tests) that,
 // once x_m repeats x_{m-2} to 1e-9 K, returns the point x_30 would be
 // (x_m for an even number of remaining updates, x_{m-1} for an odd one)
-// instead of doing the remaining updates.  Off the cycle it is get_T.
+// instead of doing the remaining updates.  Off the cycle it is get_T.  Its
+// updates evaluate the energy and c_v with the reciprocal molar masses (one
+// rounding each differs from get_T's; the cycle point it returns agrees with
+// get_T's to that rounding).
 template <int NS>
 __device__ inline double get_T_gap(const DevParams* P, double ein, const double* rhoi) {
   const double tol = 1e-3;
@@ -314,8 +317,8 @@ __device__ inline double get_T_gap(const DevParams* P, double ein, const double*
   if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
   double x1 = 0.0, x2 = 0.0;  // t1 one and two updates ago
   for (int m = 0; m < 30; ++m) {
-    const double e1 = internal_energy<NS>(P, t1, rhoi);
-    const double cv = cv_mix<NS>(P, t1, rhoi);
+    const double e1 = internal_energy<NS, true>(P, t1, rhoi);
+    const double cv = cv_mix<NS, true>(P, t1, rhoi);
     double dt = (ein - e1) / cv;
     if (dt > 100.0) {
       dt = 100.0;

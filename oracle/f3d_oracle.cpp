@@ -630,7 +630,8 @@ double or_get_T(const or_dims* d, const or_thermo* th, double ein, const double*
 // of the two fits' continuations until its updates run out; once x_m repeats
 // x_{m-2} to 1e-9 K the point x_maxiter is returned by parity (x_m for an
 // even number of remaining updates, x_{m-1} for an odd one).  The library's
-// get_T_gap; pinned by agreement with or_get_T inside the jump (tests/).
+// get_T_gap, whose updates take the energy and c_v with 1/W_n multiplied
+// (cv_t<RCP>); pinned by agreement with or_get_T inside the jump (tests/).
 double or_get_T_gap(const or_dims* d, const or_thermo* th, double ein, const double* rhoi,
                     double tol, int32_t maxiter, double tmin, double tmax) {
   Grid g(d);
@@ -644,8 +645,8 @@ double or_get_T_gap(const or_dims* d, const or_thermo* th, double ein, const dou
   if (t1 < tmin || t1 > tmax) t1 = tmin + (tmax - tmin) / (emax - emin) * (ein - emin);
   double x1 = 0.0, x2 = 0.0;
   for (int m = 0; m < maxiter; ++m) {
-    double e1 = or_internal_energy(d, th, t1, rhoi);
-    double cv = or_cv(d, th, t1, rhoi);
+    double e1 = internal_energy_t<double, true>(g, th, t1, rhoi);
+    double cv = cv_t<double, true>(g, th, t1, rhoi);
     double dt = (ein - e1) / cv;
     if (dt > 100.0) {
       dt = 100.0;
