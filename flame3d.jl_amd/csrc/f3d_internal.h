@@ -97,6 +97,7 @@ struct f3d_ctx {
   int32_t* nn_fblist = nullptr;   // rejected cell ids in cell order [Nx*Ny*Nz]
   int32_t* nn_fbtotal = nullptr;  // number of rejected cells
   float* nn_out = nullptr;        // the network's raw outputs [Ns][Nx*Ny*Nz] for the epilogue
+  double* nn_cand = nullptr;      // the epilogue's candidate states [Ns + 3][Nx*Ny*Nz]
   // f3d_nn_timing: events around the last f3d_nn_chem's network kernel, its
   // gate epilogue kernel, its scan / compaction and its Newton fallback, on
   // the call's stream

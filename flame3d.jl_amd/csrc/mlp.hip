@@ -527,175 +527,210 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
+// The epilogue runs as two kernels so each keeps its registers at 4 waves
+// per SIMD: k_nn_pred (gates 2-4, the predicted state and its temperature)
+// stores the candidate state of every cell still accepted (rho_k', T', e',
+// the energy change) to `cand` ([NS + 3][cells], FP64); k_nn_gate evaluates
+// the rate-based gates 5-7 on it and commits or rejects.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+    k_nn_pred(const double* __restrict__ Q, const double* __restrict__ rhoi,
+              const DevParams* __restrict__ P, const NNDev* __restrict__ C, Geom g,
+              const float* __restrict__ out, double* __restrict__ cand,
+              int8_t* __restrict__ status) {
+  const long ncell = (long)g.Nx * g.Ny * g.Nz;
+  const long id = gid();
+  if (id >= ncell) return;
+  int code = status[id];  // 0 inactive, 1 candidate, 2 Mahalanobis
+  if (code != 1) return;
+  const long NP = g.NP;
+  const int a = (int)(id % g.Nx);
+  const int t = (int)(id / g.Nx);
+  const int b = t % g.Ny, c = t / g.Ny;
+  const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
+  const double T = Q[p + 5 * NP];
+  const double rho = Q[p];
+  double r[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r[n] = rhoi[p + n * NP];
+  // post_predict (reactions.jl:26-56) on the de-normalized outputs
+  const double rinv = 1.0 / fmax(rho, kEps);
+  const double lam = (double)C->boxcox_lambda;
+  double Y[NS], dY0[NS], dY[NS], S[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    Y[n] = fmax(r[n] * rinv, 0.0);
+    const double o = (double)(out[(long)n * ncell + id] * C->out_std[n] + C->out_mean[n]);
+    if (C->label_mode == 0) {
+      dY0[n] = o;
+    } else if (C->label_mode == 2) {
+      dY0[n] = (double)C->label_scale[n] * sinh(o);
+    } else if (C->label_mode == 3) {
+      const double sk = fmin((double)C->label_scale[n],
+                             fmax((double)C->label_floor, (double)C->label_rel * Y[n]));
+      dY0[n] = sk * sinh(o);
+      S[n] = (sk + fabs(dY0[n])) * (sk + fabs(dY0[n])) + 1e-60;
+    } else {
+      const double base = lam * ((pow(Y[n], lam) - 1.0) / lam + o) + 1.0;
+      dY0[n] = (base > 0.0 ? pow(base, 1.0 / lam) : 0.0) - Y[n];
+    }
+  }
+  // 2. element balance of the raw prediction (net change of each element
+  //    against the gross turnover of all elements, sum_e sum_k a_ek |dY_k| / W_k,
+  //    so an inert species' noise counts against the whole change), then
+  //    the projection
+  bool eok = true;
+  {
+    double gross = 0.0, worst = 0.0;
+    for (int e = 0; e < C->n_elem; ++e) {
+      double num = 0.0;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const double ae = C->elem_mw[e][n];
+        num += ae * dY0[n];
+        gross += ae * fabs(dY0[n]);
+      }
+      worst = fmax(worst, fabs(num));
+    }
+    if (gross > 0.0 && worst > (double)C->elem_tol * gross) eok = false;
+  }
+  if (C->label_mode == 3) {
+    // per-cell weights S_k = (s_k + |dY0_k|)^2: dY = dY0 - S E^T (E S E^T)^-1 E dY0
+    if (!project_weighted<NS>(C, P, S, dY0, dY)) eok = false;
+  } else {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < NS; ++m) v += C->proj[n][m] * dY0[m];
+      dY[n] = v;
+    }
+  }
+  if (!eok) code = 3;
+  bool pok = true;
+#pragma unroll
+  for (int n = 0; n < NS; ++n)  // 3. positivity
+    if (Y[n] + dY[n] < -(double)C->neg_tol) pok = false;
+  if (code == 1 && !pok) code = 4;
+  // 4. fast chemistry: the temperature change the prediction implies
+  if (code == 1 && C->dT_max > 0.0f) {
+    double esum = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) esum += fabs(P->hfm[n] * dY[n]);
+    if (!(esum * rho <= (double)C->dT_max * cv_mix<NS, true>(P, T, r))) code = 5;
+  }
+  if (code == 1) {
+    double rn[NS], dei = 0.0, sr = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      rn[n] = fmax(Y[n] + dY[n], 0.0) * rho;
+      dei += -P->hfm[n] * (rn[n] - r[n]);
+    }
+    // the state chem_epilogue would write, evaluated before committing
+    const double rhoc = fmax(rho, kEps);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) sr += rn[n];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) rn[n] *= rhoc / sr;
+    const double ein = Q[p + 6 * NP] + dei;
+    const double Tn = fmax(get_T_from<NS>(P, ein, rn, T), kEps);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) cand[(long)n * ncell + id] = rn[n];
+    cand[(long)NS * ncell + id] = Tn;
+    cand[(long)(NS + 1) * ncell + id] = ein;
+    cand[(long)(NS + 2) * ncell + id] = dei;
+  }
+  if (code != 1) status[id] = (int8_t)code;
+}
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
-    k_nn_epi(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
-             const DevParams* __restrict__ P, const NNDev* __restrict__ C, Geom g, double dt,
-             const float* __restrict__ out, int32_t* __restrict__ tile_count,
-             int8_t* __restrict__ status) {
+    k_nn_gate(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
+              const DevParams* __restrict__ P, const NNDev* __restrict__ C, Geom g, double dt,
+              const double* __restrict__ cand, int32_t* __restrict__ tile_count,
+              int8_t* __restrict__ status) {
   const long ncell = (long)g.Nx * g.Ny * g.Nz;
   const long id = gid();  // a wave holds the 64 cells of one tile
   const long NP = g.NP;
-  int code = (id < ncell) ? status[id] : 0;  // 0 inactive, 1 candidate, 2 Mahalanobis
+  int code = (id < ncell) ? status[id] : 0;
   if (code == 1) {
     const int a = (int)(id % g.Nx);
     const int t = (int)(id / g.Nx);
     const int b = t % g.Ny, c = t / g.Ny;
     const long p = (a + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
-    const double T = Q[p + 5 * NP];
-    const double rho = Q[p];
-    double r[NS];
+    double rn[NS];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) r[n] = rhoi[p + n * NP];
-    // post_predict (reactions.jl:26-56) on the de-normalized outputs
-    const double rinv = 1.0 / fmax(rho, kEps);
-    const double lam = (double)C->boxcox_lambda;
-    double Y[NS], dY0[NS], dY[NS], S[NS];
+    for (int n = 0; n < NS; ++n) rn[n] = cand[(long)n * ncell + id];
+    const double Tn = cand[(long)NS * ncell + id];
+    const double rhoc = fmax(Q[p], kEps);
+    // 5. backward-Euler residual at (rho', T'): its largest species entry,
+    //    and the temperature error its formation energies imply
+    const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;
+    if (res_on || C->chi_max > 0.0f || C->chain_max > 0.0f) {
+      double sc[NS], wdot[NS], Adum[NS][NS], Mc[3][3];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      Y[n] = fmax(r[n] * rinv, 0.0);
-      const double o = (double)(out[(long)n * ncell + id] * C->out_std[n] + C->out_mean[n]);
-      if (C->label_mode == 0) {
-        dY0[n] = o;
-      } else if (C->label_mode == 2) {
-        dY0[n] = (double)C->label_scale[n] * sinh(o);
-      } else if (C->label_mode == 3) {
-        const double sk = fmin((double)C->label_scale[n],
-                               fmax((double)C->label_floor, (double)C->label_rel * Y[n]));
-        dY0[n] = sk * sinh(o);
-        S[n] = (sk + fabs(dY0[n])) * (sk + fabs(dY0[n])) + 1e-60;
+      for (int n = 0; n < NS; ++n) sc[n] = rn[n] * P->cmw[n];
+      if (C->chain_max > 0.0f) {
+        const int cidx[3] = {C->chain_idx[0], C->chain_idx[1], C->chain_idx[2]};
+        production_rate_block3<NS>(P, sc, Tn, cidx, wdot, Mc);
       } else {
-        const double base = lam * ((pow(Y[n], lam) - 1.0) / lam + o) + 1.0;
-        dY0[n] = (base > 0.0 ? pow(base, 1.0 / lam) : 0.0) - Y[n];
+        production_rate<NS, false>(P, sc, Tn, Adum, wdot);
       }
-    }
-    // 2. element balance of the raw prediction (net change of each element
-    //    against the gross turnover of all elements, sum_e sum_k a_ek |dY_k| / W_k,
-    //    so an inert species' noise counts against the whole change), then
-    //    the projection
-    bool eok = true;
-    {
-      double gross = 0.0, worst = 0.0;
-      for (int e = 0; e < C->n_elem; ++e) {
-        double num = 0.0;
+      const double cvn = cv_mix<NS, true>(P, Tn, rn);
+      if (res_on) {
+        double rmax = 0.0, esum = 0.0;
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
-          const double ae = C->elem_mw[e][n];
-          num += ae * dY0[n];
-          gross += ae * fabs(dY0[n]);
+          const double R = rn[n] - rhoi[p + n * NP] - wdot[n] * P->mw[n] * 1e6 * dt;
+          rmax = fmax(rmax, fabs(R));
+          esum += fabs(P->hfm[n] * R);
         }
-        worst = fmax(worst, fabs(num));
+        if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
+        if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cvn)) code = 6;
       }
-      if (gross > 0.0 && worst > (double)C->elem_tol * gross) eok = false;
-    }
-    if (C->label_mode == 3) {
-      // per-cell weights S_k = (s_k + |dY0_k|)^2: dY = dY0 - S E^T (E S E^T)^-1 E dY0
-      if (!project_weighted<NS>(C, P, S, dY0, dY)) eok = false;
-    } else {
+      // 6. thermal explosiveness: the heat-release response to T over one step
+      if (code == 1 && C->chi_max > 0.0f) {
+        double w2[NS];
+        const double hT = 1e-4 * Tn;
+        production_rate<NS, false>(P, sc, Tn + hT, Adum, w2);
+        double dq = 0.0;
 #pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        double v = 0.0;
-#pragma unroll
-        for (int m = 0; m < NS; ++m) v += C->proj[n][m] * dY0[m];
-        dY[n] = v;
+        for (int n = 0; n < NS; ++n)
+          dq -= P->lo[n][5] * P->Ru * 1e6 * (w2[n] - wdot[n]);  // h_f,k W_k d(wdot_k)
+        if (!(dt * dq / hT <= (double)C->chi_max * cvn)) code = 7;
       }
-    }
-    if (code == 1 && !eok) code = 3;
-    bool pok = true;
+      // 7. chain explosiveness: the 3x3 block M = dt d wdot / d c over the
+      //    chain carriers (from production_rate_block3 above), rejected
+      //    unless every eigenvalue of M has real part < chain_max:
+      //    Routh-Hurwitz on N = chain_max I - M (tr N > 0, det N > 0,
+      //    tr N * m2(N) > det N, m2 = sum of the principal 2x2 minors)
+      if (code == 1 && C->chain_max > 0.0f) {
+        double Nm[3][3];
 #pragma unroll
-    for (int n = 0; n < NS; ++n)  // 3. positivity
-      if (Y[n] + dY[n] < -(double)C->neg_tol) pok = false;
-    if (code == 1 && !pok) code = 4;
-    // 4. fast chemistry: the temperature change the prediction implies
-    if (code == 1 && C->dT_max > 0.0f) {
-      double esum = 0.0;
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int n = 0; n < NS; ++n) esum += fabs(P->hfm[n] * dY[n]);
-      if (!(esum * rho <= (double)C->dT_max * cv_mix<NS, true>(P, T, r))) code = 5;
+          for (int j = 0; j < 3; ++j)
+            Nm[i][j] = (i == j ? (double)C->chain_max : 0.0) - dt * Mc[i][j];
+        const double tr = Nm[0][0] + Nm[1][1] + Nm[2][2];
+        const double m2 = Nm[0][0] * Nm[1][1] - Nm[0][1] * Nm[1][0] +
+                          Nm[0][0] * Nm[2][2] - Nm[0][2] * Nm[2][0] +
+                          Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1];
+        const double det = Nm[0][0] * (Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1]) -
+                           Nm[0][1] * (Nm[1][0] * Nm[2][2] - Nm[1][2] * Nm[2][0]) +
+                           Nm[0][2] * (Nm[1][0] * Nm[2][1] - Nm[1][1] * Nm[2][0]);
+        if (!(tr > 0.0 && det > 0.0 && tr * m2 > det)) code = 8;
+      }
     }
     if (code == 1) {
-      double rn[NS], dei = 0.0, sr = 0.0;
+      U[p + 4 * NP] += cand[(long)(NS + 2) * ncell + id];
+      Q[p + 4 * NP] = fmax(pmixture<NS, true>(P, Tn, rn), kEps);
+      Q[p + 5 * NP] = Tn;
+      Q[p + 6 * NP] = cand[(long)(NS + 1) * ncell + id];
 #pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        rn[n] = fmax(Y[n] + dY[n], 0.0) * rho;
-        dei += -P->hfm[n] * (rn[n] - r[n]);
-      }
-      // the state chem_epilogue would write, evaluated before committing
-      const double rhoc = fmax(rho, kEps);
-#pragma unroll
-      for (int n = 0; n < NS; ++n) sr += rn[n];
-#pragma unroll
-      for (int n = 0; n < NS; ++n) rn[n] *= rhoc / sr;
-      const double ein = Q[p + 6 * NP] + dei;
-      const double Tn = fmax(get_T_from<NS>(P, ein, rn, T), kEps);
-      // 5. backward-Euler residual at (rho', T'): its largest species entry,
-      //    and the temperature error its formation energies imply
-      const bool res_on = C->res_tol > 0.0f || C->res_T_tol > 0.0f;
-      if (res_on || C->chi_max > 0.0f || C->chain_max > 0.0f) {
-        double sc[NS], wdot[NS], Adum[NS][NS], Mc[3][3];
-#pragma unroll
-        for (int n = 0; n < NS; ++n) sc[n] = rn[n] * P->cmw[n];
-        if (C->chain_max > 0.0f) {
-          const int cidx[3] = {C->chain_idx[0], C->chain_idx[1], C->chain_idx[2]};
-          production_rate_block3<NS>(P, sc, Tn, cidx, wdot, Mc);
-        } else {
-          production_rate<NS, false>(P, sc, Tn, Adum, wdot);
-        }
-        const double cvn = cv_mix<NS, true>(P, Tn, rn);
-        if (res_on) {
-          double rmax = 0.0, esum = 0.0;
-#pragma unroll
-          for (int n = 0; n < NS; ++n) {
-            const double R = rn[n] - r[n] - wdot[n] * P->mw[n] * 1e6 * dt;
-            rmax = fmax(rmax, fabs(R));
-            esum += fabs(P->hfm[n] * R);
-          }
-          if (C->res_tol > 0.0f && !(rmax <= (double)C->res_tol * rhoc)) code = 6;
-          if (C->res_T_tol > 0.0f && !(esum <= (double)C->res_T_tol * cvn)) code = 6;
-        }
-        // 6. thermal explosiveness: the heat-release response to T over one step
-        if (code == 1 && C->chi_max > 0.0f) {
-          double w2[NS];
-          const double hT = 1e-4 * Tn;
-          production_rate<NS, false>(P, sc, Tn + hT, Adum, w2);
-          double dq = 0.0;
-#pragma unroll
-          for (int n = 0; n < NS; ++n)
-            dq -= P->lo[n][5] * P->Ru * 1e6 * (w2[n] - wdot[n]);  // h_f,k W_k d(wdot_k)
-          if (!(dt * dq / hT <= (double)C->chi_max * cvn)) code = 7;
-        }
-        // 7. chain explosiveness: the 3x3 block M = dt d wdot / d c over the
-        //    chain carriers (from production_rate_block3 above), rejected
-        //    unless every eigenvalue of M has real part < chain_max:
-        //    Routh-Hurwitz on N = chain_max I - M (tr N > 0, det N > 0,
-        //    tr N * m2(N) > det N, m2 = sum of the principal 2x2 minors)
-        if (code == 1 && C->chain_max > 0.0f) {
-          double Nm[3][3];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-              Nm[i][j] = (i == j ? (double)C->chain_max : 0.0) - dt * Mc[i][j];
-          const double tr = Nm[0][0] + Nm[1][1] + Nm[2][2];
-          const double m2 = Nm[0][0] * Nm[1][1] - Nm[0][1] * Nm[1][0] +
-                            Nm[0][0] * Nm[2][2] - Nm[0][2] * Nm[2][0] +
-                            Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1];
-          const double det = Nm[0][0] * (Nm[1][1] * Nm[2][2] - Nm[1][2] * Nm[2][1]) -
-                             Nm[0][1] * (Nm[1][0] * Nm[2][2] - Nm[1][2] * Nm[2][0]) +
-                             Nm[0][2] * (Nm[1][0] * Nm[2][1] - Nm[1][1] * Nm[2][0]);
-          if (!(tr > 0.0 && det > 0.0 && tr * m2 > det)) code = 8;
-        }
-      }
-      if (code == 1) {
-        U[p + 4 * NP] += dei;
-        Q[p + 4 * NP] = fmax(pmixture<NS, true>(P, Tn, rn), kEps);
-        Q[p + 5 * NP] = Tn;
-        Q[p + 6 * NP] = ein;
-#pragma unroll
-        for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = rn[n];
-      }
+      for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = rn[n];
+    } else {
+      status[id] = (int8_t)code;
     }
-    status[id] = (int8_t)code;
   }
   // rejected cells of this 64-cell tile (status >= 2); the ordered list the
   // Newton pass runs over is built from these counts (k_nn_scan, k_nn_compact)
@@ -899,11 +934,15 @@ static int static_scratch(float** out) {
 }
 
 void nn_free(f3d_ctx* c) {
-  void* ptrs[] = {c->nn_master, c->nn_m1, c->nn_m2, c->nn_Wb, c->nn_Wtb, c->nn_partial,
-                  c->nn_cfg, c->nn_status, c->nn_fbcount, c->nn_fblist, c->nn_fbtotal,
-                  c->nn_out};
-  for (void* q : ptrs)
-    if (q) (void)hipFree(q);
+  void** ptrs[] = {(void**)&c->nn_master, (void**)&c->nn_m1, (void**)&c->nn_m2,
+                   (void**)&c->nn_Wb, (void**)&c->nn_Wtb, (void**)&c->nn_partial,
+                   (void**)&c->nn_cfg, (void**)&c->nn_status, (void**)&c->nn_fbcount,
+                   (void**)&c->nn_fblist, (void**)&c->nn_fbtotal, (void**)&c->nn_out,
+                   (void**)&c->nn_cand};
+  for (void** q : ptrs) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
   for (hipEvent_t& e : c->nn_ev)
     if (e) {
       (void)hipEventDestroy(e);
@@ -1014,7 +1053,8 @@ int f3d_nn_init(f3d_ctx* c, const f3d_nn_config* cfg) {
                hipMalloc(&c->nn_fbcount, sizeof(int32_t) * (ntile + nchunk)) == hipSuccess &&
                hipMalloc(&c->nn_fblist, sizeof(int32_t) * ncell) == hipSuccess &&
                hipMalloc(&c->nn_fbtotal, sizeof(int32_t)) == hipSuccess &&
-               hipMalloc(&c->nn_out, sizeof(float) * ncell * ns) == hipSuccess;
+               hipMalloc(&c->nn_out, sizeof(float) * ncell * ns) == hipSuccess &&
+               hipMalloc(&c->nn_cand, sizeof(double) * ncell * (ns + 3)) == hipSuccess;
     if (!okm) {
       nn_free(c);
       set_error("f3d_nn_init: device allocation failed");
@@ -1177,12 +1217,17 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
                        c->dparams, (const NNDev*)c->nn_cfg, (const __bf16*)c->nn_Wb, c->nn_master,
                        c->g, c->nn_out, st);
   if (timed) (void)hipEventRecord(c->nn_ev[1], s);
-  if (c->g.Ns == 9)
-    hipLaunchKernelGGL((k_nn_epi<9>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
-                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_out, c->nn_fbcount, st);
-  else
-    hipLaunchKernelGGL((k_nn_epi<5>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
-                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_out, c->nn_fbcount, st);
+  if (c->g.Ns == 9) {
+    hipLaunchKernelGGL((k_nn_pred<9>), dim3(nb), dim3(kBlock), 0, s, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, c->nn_out, c->nn_cand, st);
+    hipLaunchKernelGGL((k_nn_gate<9>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
+  } else {
+    hipLaunchKernelGGL((k_nn_pred<5>), dim3(nb), dim3(kBlock), 0, s, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, c->nn_out, c->nn_cand, st);
+    hipLaunchKernelGGL((k_nn_gate<5>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
+  }
   if (timed) (void)hipEventRecord(c->nn_ev[2], s);
   // tile counts -> tile offsets; the chunk sums sit after the tile counts
   const long nchunk = (ntile + kScanChunk - 1) / kScanChunk;
