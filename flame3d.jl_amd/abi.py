@@ -59,6 +59,7 @@ NN_NPARAM = 12512  # F3D_NN_NPARAM
 SENSOR_PRESSURE, SENSOR_DUCROS = 0, 1  # F3D_SENSOR_*
 RECON_REFERENCE, RECON_WENO7 = 0, 1    # F3D_RECON_*
 STAGE_FUSED, STAGE_GRANULAR = 0, 1     # F3D_STAGE_*
+KIN_RUNTIME, KIN_H2_LIDRYER, KIN_AIR5 = 0, 1, 2  # F3D_KIN_*
 # Ducros switch (UP7 below theta = 0.1, WENO5-Z above, never NND)
 DUCROS_SCHEME = dict(sensor_kind=SENSOR_DUCROS, hybrid_phi1=0.1, hybrid_phi2=2.0)
 
@@ -83,6 +84,7 @@ _PROTOS = {
                    C.POINTER(_vp)],
     "f3d_destroy": [_vp],
     "f3d_real_size": [_vp],
+    "f3d_kinetics_source": [_vp],
     "f3d_flux_split": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "f3d_species_split": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "f3d_shock_sensor": [_vp, _vp, _vp, _vp],
@@ -211,6 +213,14 @@ class Context:
                                 C.byref(h)), "f3d_create")
         self.h = h
         self._metrics_keep = None
+
+    def kinetics_source(self) -> int:
+        """F3D_KIN_*: compiled-in reaction tables of a shipped mechanism, or
+        the run-time parameter block."""
+        k = self.lib.f3d_kinetics_source(self.h)
+        if k < 0:
+            _rc(k, "f3d_kinetics_source")
+        return k
 
     def close(self):
         if getattr(self, "h", None):

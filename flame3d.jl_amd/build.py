@@ -3,7 +3,8 @@
     python flame3d.jl_amd/build.py [--jobs N]
 
 Every .hip file under csrc/ is compiled to an object in _build/ (in
-parallel), then linked into flame3d.jl_amd/libflame3d.so.  Flags:
+parallel), then linked into flame3d.jl_amd/libflame3d.so, after
+mechanisms/gen_mech_tables.py has written csrc/f3d_mech_tables.h.  Flags:
   --offload-arch=gfx950            CDNA4 only, no other targets
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                    no packed FP32 instructions
@@ -39,6 +40,9 @@ FILE_FLAGS = {"chem.hip": ["-ffp-contract=off"]}
 
 
 def build(jobs: int = 8, verbose: bool = False) -> str:
+    # the shipped mechanisms' reaction tables (csrc/f3d_mech_tables.h)
+    subprocess.run([sys.executable, os.path.join(HERE, "mechanisms", "gen_mech_tables.py")],
+                   check=True)
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objs = []

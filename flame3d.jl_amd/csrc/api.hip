@@ -6,6 +6,7 @@
 #include <string>
 
 #include "f3d_internal.h"
+#include "f3d_mech_tables.h"
 
 namespace f3d {
 
@@ -113,6 +114,30 @@ static int fill_params(DevParams& P, const f3d_dims* d, const f3d_thermo* th,
   P.SWeps2_f = (float)P.SWeps2;
   for (int c = 0; c < 7; ++c) P.UP7_f[c] = (float)P.UP7[c];
   return F3D_OK;
+}
+
+// The compiled-in reaction tables (f3d_mech_tables.h) that equal the
+// context's kinetics in every entry, if any.
+template <class T>
+static bool same_kinetics(const DevParams& P) {
+  if (P.Ns != T::NS || P.Nr != T::NR) return false;
+  for (int r = 0; r < T::NR; ++r) {
+    if (P.rtype[r] != T::rtype[r] || P.sgm[r] != T::sgm[r]) return false;
+    for (int k = 0; k < 3; ++k)
+      if (P.Arr[r][k] != T::Arr[r][k] || P.loP[r][k] != T::loP[r][k]) return false;
+    for (int k = 0; k < 4; ++k)
+      if (P.Troe[r][k] != T::Troe[r][k]) return false;
+    for (int n = 0; n < T::NS; ++n)
+      if (P.vf[r][n] != T::vf[r][n] || P.vr[r][n] != T::vr[r][n] || P.ef[r][n] != T::ef[r][n])
+        return false;
+  }
+  return true;
+}
+
+static int kinetics_table(const DevParams& P) {
+  if (same_kinetics<MechH2LiDryer>(P)) return F3D_KIN_H2_LIDRYER;
+  if (same_kinetics<MechAir5>(P)) return F3D_KIN_AIR5;
+  return F3D_KIN_RUNTIME;
 }
 
 static size_t real_size(const f3d_ctx* c) { return c->precision == F3D_FP64 ? 8 : 4; }
@@ -256,6 +281,7 @@ int f3d_create(const f3d_dims* d, const f3d_thermo* th, const f3d_kinetics* kin,
     delete c;
     return rc;
   }
+  c->kin_table = kinetics_table(c->host_params);
   if (hipMalloc(&c->dparams, sizeof(DevParams)) != hipSuccess) {
     delete c;
     set_error("f3d_create: device allocation of parameters failed");
@@ -295,6 +321,8 @@ static const float* registered_float(const f3d_ctx* c, const double* m) {
 }
 
 int f3d_real_size(const f3d_ctx* c) { return c ? (int)real_size(c) : F3D_ERR_ARG; }
+
+int f3d_kinetics_source(const f3d_ctx* c) { return c ? c->kin_table : F3D_ERR_ARG; }
 
 int f3d_flux_split(f3d_ctx* c, const double* Q, void* Fp, void* Fm, const double* Ax,
                    const double* Ay, const double* Az, const int64_t* tag, hipStream_t s) {

@@ -9,6 +9,7 @@
 // are wave-uniform branches that skip the zero terms the reference adds.
 #include "f3d_chem.h"
 #include "f3d_internal.h"
+#include "f3d_mech_tables.h"
 
 namespace f3d {
 
@@ -24,7 +25,7 @@ constexpr int kNewtonMaxSub = 64;
 // are multiplied where the reference's point-implicit step divides.  Returns false when the iterate leaves the admissible set
 // (non-finite, T outside [T_lo, T_hi], rho_k < -rho_tot) or maxiter
 // iterations pass without convergence; `it` counts the iterations taken.
-template <int NS>
+template <int NS, class K>
 __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS], double& T,
                                 double h, double E0, const double (&hf)[NS], double rho_tot,
                                 double T_lo, double T_hi, int maxiter, double rtol, double ttol,
@@ -41,7 +42,7 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
     // and turned into I - W_n/W_l h dwdot/dc in place (one matrix in registers)
     double A[N1][N1], rhs[N1], dx[N1];
     // rate Jacobian in A[0..NS-1][0..NS-1], d wdot / dT in A[0..NS-1][NS]
-    production_rate<NS, true, true, N1, true, true>(P, sc, T, A, wdot);
+    production_rate<NS, true, true, N1, true, true, K>(P, sc, T, A, wdot);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const double src = wdot[n] * P->mw[n] * 1e6 * h;
@@ -98,8 +99,9 @@ __device__ inline bool be_solve(const DevParams* __restrict__ P, double (&r)[NS]
   return false;
 }
 
-// mode 0: eval_gpu_stiff, 1: eval_gpu, 2: backward-Euler Newton
-template <int NS, int MODE>
+// mode 0: eval_gpu_stiff, 1: eval_gpu, 2: backward-Euler Newton.  K: the
+// reaction parameters, from the parameter block or compiled in (KinCT).
+template <int NS, int MODE, class K = KinRT>
 __global__ void __launch_bounds__(kBlock)
     k_chem(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi, double dt,
            int maxiter, double rtol, double ttol, int32_t* __restrict__ iters,
@@ -127,7 +129,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (MODE == 1) {
     double A[NS][NS];
-    production_rate<NS, false>(P, sc, T0, A, wdot);
+    production_rate<NS, false, false, NS, false, false, K>(P, sc, T0, A, wdot);
     double dei = 0;
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(kBlock)
     chem_epilogue<NS>(P, U, Q, rhoi, p, NP, r, dei);
   } else if (MODE == 0) {
     double A[NS][NS], bb[NS], x[NS];
-    production_rate<NS, true>(P, sc, T0, A, wdot);
+    production_rate<NS, true, false, NS, false, false, K>(P, sc, T0, A, wdot);
 #pragma unroll
     for (int l = 0; l < NS; ++l)
 #pragma unroll
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(kBlock)
       ok = true;
       const double h = dt / nsub;
       for (int k = 0; k < nsub && ok; ++k)
-        ok = be_solve<NS>(P, r, T, h, E0, hf, rho_tot, T_lo, T_hi, maxiter, rtol, ttol, it);
+        ok = be_solve<NS, K>(P, r, T, h, E0, hf, rho_tot, T_lo, T_hi, maxiter, rtol, ttol, it);
     }
     if (!ok) {  // no admissible solution: the reactor is left unreacted
 #pragma unroll
@@ -196,22 +198,45 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <int NS>
-static void chem_ns(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,
-                    int maxiter, double rtol, double ttol, int32_t* iters, const int64_t* tag,
-                    hipStream_t s, const int32_t* list, long nlist) {
+template <int NS, class K>
+static void chem_k(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,
+                   int maxiter, double rtol, double ttol, int32_t* iters, const int64_t* tag,
+                   hipStream_t s, const int32_t* list, long nlist) {
   const long n = list ? nlist : (long)c->g.Nx * c->g.Ny * c->g.Nz;
   if (n <= 0) return;
   const int bug = c->bug_compat & 1;
   if (mode == 0)
-    hipLaunchKernelGGL((k_chem<NS, 0>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
+    hipLaunchKernelGGL((k_chem<NS, 0, K>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
                        maxiter, rtol, ttol, iters, tag, c->dparams, c->g, bug, list, n);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_chem<NS, 1>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
+    hipLaunchKernelGGL((k_chem<NS, 1, K>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
                        maxiter, rtol, ttol, iters, tag, c->dparams, c->g, bug, list, n);
   else
-    hipLaunchKernelGGL((k_chem<NS, 2>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
+    hipLaunchKernelGGL((k_chem<NS, 2, K>), dim3(nblocks(n)), dim3(kBlock), 0, s, U, Q, rhoi, dt,
                        maxiter, rtol, ttol, iters, tag, c->dparams, c->g, bug, list, n);
+}
+
+// the compiled-in tables of the context's mechanism when it is one of the
+// shipped ones (f3d_create compares every entry), else the parameter block
+template <int NS>
+static void chem_ns(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,
+                    int maxiter, double rtol, double ttol, int32_t* iters, const int64_t* tag,
+                    hipStream_t s, const int32_t* list, long nlist) {
+  if constexpr (NS == MechH2LiDryer::NS) {
+    if (c->kin_table == F3D_KIN_H2_LIDRYER) {
+      chem_k<NS, KinCT<MechH2LiDryer>>(c, mode, U, Q, rhoi, dt, maxiter, rtol, ttol, iters, tag,
+                                       s, list, nlist);
+      return;
+    }
+  }
+  if constexpr (NS == MechAir5::NS) {
+    if (c->kin_table == F3D_KIN_AIR5) {
+      chem_k<NS, KinCT<MechAir5>>(c, mode, U, Q, rhoi, dt, maxiter, rtol, ttol, iters, tag, s,
+                                  list, nlist);
+      return;
+    }
+  }
+  chem_k<NS, KinRT>(c, mode, U, Q, rhoi, dt, maxiter, rtol, ttol, iters, tag, s, list, nlist);
 }
 
 int launch_chem(f3d_ctx* c, int mode, double* U, double* Q, double* rhoi, double dt,

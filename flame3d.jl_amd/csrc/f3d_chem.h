@@ -100,6 +100,58 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
   }
 }
 
+// Reaction parameters of the rate evaluation.  KinRT reads them from the
+// uniform parameter block (any mechanism: scalar loads and wave-uniform
+// branches on the stoichiometry); KinCT<T> takes them from the compile-time
+// tables T of a shipped mechanism (f3d_mech_tables.h), so the reaction loop
+// unrolls and every stoichiometric test and zero term is resolved by the
+// compiler.  Both perform the same arithmetic on the same values.
+struct KinRT {
+  const DevParams* __restrict__ P;
+  __device__ explicit KinRT(const DevParams* p) : P(p) {}
+  static constexpr bool kConst = false;
+  __device__ int nr() const { return P->Nr; }
+  __device__ int rtype(int n) const { return P->rtype[n]; }
+  __device__ int sgm(int n) const { return P->sgm[n]; }
+  __device__ int vf(int n, int m) const { return P->vf[n][m]; }
+  __device__ int vr(int n, int m) const { return P->vr[n][m]; }
+  __device__ double Arr(int n, int k) const { return P->Arr[n][k]; }
+  __device__ double ef(int n, int m) const { return P->ef[n][m]; }
+  __device__ double loP(int n, int k) const { return P->loP[n][k]; }
+  __device__ double Troe(int n, int k) const { return P->Troe[n][k]; }
+};
+template <class T>
+struct KinCT {
+  __device__ explicit constexpr KinCT(const DevParams*) {}
+  static constexpr bool kConst = true;
+  __device__ constexpr int nr() const { return T::NR; }
+  __device__ constexpr int rtype(int n) const { return T::rtype[n]; }
+  __device__ constexpr int sgm(int n) const { return T::sgm[n]; }
+  __device__ constexpr int vf(int n, int m) const { return T::vf[n][m]; }
+  __device__ constexpr int vr(int n, int m) const { return T::vr[n][m]; }
+  __device__ constexpr double Arr(int n, int k) const { return T::Arr[n][k]; }
+  __device__ constexpr double ef(int n, int m) const { return T::ef[n][m]; }
+  __device__ constexpr double loP(int n, int k) const { return T::loP[n][k]; }
+  __device__ constexpr double Troe(int n, int k) const { return T::Troe[n][k]; }
+};
+
+// for (n = 0; n < K.nr(); ++n) body(n): unrolled for compile-time tables
+template <class K, class F>
+__device__ __forceinline__ void for_reactions(const K& Kp, F&& body) {
+  if constexpr (K::kConst) {
+#pragma unroll
+    for (int n = 0; n < Kp.nr(); ++n) {
+      body(n);
+      // one reaction at a time: interleaving the unrolled reactions would
+      // hold several reactions' temporaries next to the Jacobian
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    const int Nr = Kp.nr();
+    for (int n = 0; n < Nr; ++n) body(n);
+  }
+}
+
 // vproductionRate(_Jac), reactions.jl:311-460.  A[l][m] accumulates
 // Arate[l, m] = d wdot_l / d c_m; the reference neglects the third-body and
 // T dependence, FULL (Newton mode) adds the [M] derivative of three-body and
@@ -113,9 +165,11 @@ __device__ inline void gibbs(const DevParams* __restrict__ P, double T, double l
 // and of F through Fcent(T) and Pr(T).
 // RCP (Newton mode): the reverse rate constant as kf (RsT)^-sgm exp(-dg)
 // with 1/RsT formed once, instead of kf / Kc per reaction.
-template <int NS, bool JAC, bool FULL = false, int NA = NS, bool TDER = false, bool RCP = false>
+template <int NS, bool JAC, bool FULL = false, int NA = NS, bool TDER = false, bool RCP = false,
+          class K = KinRT>
 __device__ inline void production_rate(const DevParams* __restrict__ P, const double* sc, double T,
                                        double (&A)[NA][NA], double* wdot) {
+  const K Kp(P);
   const double lgT = f3d_log(T);
   const double invT = 1.0 / T;
   double gi[NS];
@@ -144,35 +198,34 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       for (int l = 0; l < NS; ++l) A[l][m] = 0.0;
     }
   }
-  const int Nr = P->Nr;
-  for (int n = 0; n < Nr; ++n) {
-    const double kf = P->Arr[n][0] * f3d_exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+  for_reactions(Kp, [&](const int n) {
+    const double kf = Kp.Arr(n, 0) * f3d_exp(Kp.Arr(n, 1) * lgT - Kp.Arr(n, 2) * invT);
     double dg = 0;
 #pragma unroll
-    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
+    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(Kp.vf(n, m) - Kp.vr(n, m));
     double kr;  // reverse rate constant kf / Kc, Kc = (RsT)^sgm exp(dg)
     if (RCP) {
-      const int sg = P->sgm[n];
+      const int sg = Kp.sgm(n);
       kr = kf * ((sg > 0 ? ipow(iRsT, sg) : ipow(RsT, -sg)) * f3d_exp(-dg));
     } else {
-      kr = kf / (ipow(RsT, P->sgm[n]) * f3d_exp(dg));
+      kr = kf / (ipow(RsT, Kp.sgm(n)) * f3d_exp(dg));
     }
     double q_f = kf;
     double q_r = kr;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      const int vf = Kp.vf(n, m), vr = Kp.vr(n, m);
       if (vf != 0) q_f *= ipow(sc[m], vf);
       if (vr != 0) q_r *= ipow(sc[m], vr);
     }
-    const int rt = P->rtype[n];
+    const int rt = Kp.rtype(n);
     double dlnq_dM = 0.0;
     double fac = 1.0;  // third-body / falloff factor multiplying both directions
     double dlnfac = 0.0;  // d ln fac / dT (TDER)
     if (rt == 2) {
       double mix = 0.0;
 #pragma unroll
-      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      for (int m = 0; m < NS; ++m) mix += Kp.ef(n, m) * sc[m];
       q_f *= mix;
       q_r *= mix;
       dlnq_dM = 1.0 / mix;
@@ -180,13 +233,13 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
     } else if (rt == 3) {
       double mix = 0.0;
 #pragma unroll
-      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
-      const double redP = mix / kf * P->loP[n][0] * f3d_exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      for (int m = 0; m < NS; ++m) mix += Kp.ef(n, m) * sc[m];
+      const double redP = mix / kf * Kp.loP(n, 0) * f3d_exp(Kp.loP(n, 1) * lgT - Kp.loP(n, 2) * invT);
       const double logPred = f3d_log10(redP);
-      const double A_ = P->Troe[n][0];
-      const double e3 = f3d_exp(-T / P->Troe[n][1]);
-      const double e1 = f3d_exp(-T / P->Troe[n][2]);
-      const double e2 = f3d_exp(-P->Troe[n][3] * invT);
+      const double A_ = Kp.Troe(n, 0);
+      const double e3 = f3d_exp(-T / Kp.Troe(n, 1));
+      const double e1 = f3d_exp(-T / Kp.Troe(n, 2));
+      const double e2 = f3d_exp(-Kp.Troe(n, 3) * invT);
       const double Fcent = (1 - A_) * e3 + A_ * e1 + e2;
       const double logFcent = f3d_log10(Fcent);
       const double troe_c = -0.4 - 0.67 * logFcent;
@@ -201,11 +254,11 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       if (TDER) {
         const double ln10 = 2.30258509299404568402;
         // d ln Pr / dT: k0 and kinf Arrhenius
-        const double dlnPr = (P->loP[n][1] - P->Arr[n][1]) * invT +
-                             (P->loP[n][2] - P->Arr[n][2]) * invT * invT;
+        const double dlnPr = (Kp.loP(n, 1) - Kp.Arr(n, 1)) * invT +
+                             (Kp.loP(n, 2) - Kp.Arr(n, 2)) * invT * invT;
         const double dL = dlnPr / ln10;
-        const double dFc = -(1 - A_) / P->Troe[n][1] * e3 - A_ / P->Troe[n][2] * e1 +
-                           P->Troe[n][3] * invT * invT * e2;
+        const double dFc = -(1 - A_) / Kp.Troe(n, 1) * e3 - A_ / Kp.Troe(n, 2) * e1 +
+                           Kp.Troe(n, 3) * invT * invT * e2;
         const double dLc = dFc / (Fcent * ln10);
         const double u = troe_c + logPred;
         const double v = troe_n - 0.14 * u;
@@ -221,11 +274,11 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       const double dqM = (q_f - q_r) * dlnq_dM;
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
-        const double e = P->ef[n][m];
+        const double e = Kp.ef(n, m);
         if (e == 0.0) continue;
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
-          const int dl = P->vr[n][l] - P->vf[n][l];
+          const int dl = Kp.vr(n, l) - Kp.vf(n, l);
           if (dl != 0) A[l][m] += dqM * e * (double)dl;
         }
       }
@@ -234,27 +287,27 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
       // exact mass-action derivatives: dq/dc_m = k nu_m c_m^(nu_m-1) prod_{j!=m} c_j^nu_j
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
-        const int vfm = P->vf[n][m], vrm = P->vr[n][m];
+        const int vfm = Kp.vf(n, m), vrm = Kp.vr(n, m);
         if (vfm == 0 && vrm == 0) continue;
         double dqf = 0.0, dqr = 0.0;
         if (vfm != 0) {
           double pr = kf * (double)vfm * ipow(sc[m], vfm - 1);
 #pragma unroll
           for (int j = 0; j < NS; ++j)
-            if (j != m && P->vf[n][j] != 0) pr *= ipow(sc[j], P->vf[n][j]);
+            if (j != m && Kp.vf(n, j) != 0) pr *= ipow(sc[j], Kp.vf(n, j));
           dqf = pr;
         }
         if (vrm != 0) {
           double pr = kr * (double)vrm * ipow(sc[m], vrm - 1);
 #pragma unroll
           for (int j = 0; j < NS; ++j)
-            if (j != m && P->vr[n][j] != 0) pr *= ipow(sc[j], P->vr[n][j]);
+            if (j != m && Kp.vr(n, j) != 0) pr *= ipow(sc[j], Kp.vr(n, j));
           dqr = pr;
         }
         const double dd = (dqf - dqr) * fac;
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
-          const int dl = P->vr[n][l] - P->vf[n][l];
+          const int dl = Kp.vr(n, l) - Kp.vf(n, l);
           if (dl != 0) A[l][m] += dd * (double)dl;
         }
       }
@@ -263,19 +316,19 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
     if (TDER) {
       double dgs = 0.0;  // sum_m (nu_f - nu_r)_m d(g_m/RT)/dT = -sum_m nu_m h_m/RT / T
 #pragma unroll
-      for (int m = 0; m < NS; ++m) dgs += hRT[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-      const double dlnkf = P->Arr[n][1] * invT + P->Arr[n][2] * invT * invT;
-      const double dlnKc = (double)P->sgm[n] * invT - dgs * invT;
+      for (int m = 0; m < NS; ++m) dgs += hRT[m] * (double)(Kp.vf(n, m) - Kp.vr(n, m));
+      const double dlnkf = Kp.Arr(n, 1) * invT + Kp.Arr(n, 2) * invT * invT;
+      const double dlnKc = (double)Kp.sgm(n) * invT - dgs * invT;
       const double dqT = q_f * (dlnkf + dlnfac) - q_r * (dlnkf - dlnKc + dlnfac);
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        const int dl = P->vr[n][l] - P->vf[n][l];
+        const int dl = Kp.vr(n, l) - Kp.vf(n, l);
         if (dl != 0) A[l][NA - 1] += dqT * (double)dl;
       }
     }
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      const int vf = Kp.vf(n, m), vr = Kp.vr(n, m);
       if (vf != vr) wdot[m] += dq * (double)(vr - vf);
       if (JAC && !FULL && (vf != 0 || vr != 0)) {
         const double invsc = 1 / (sc[m] + kEps);
@@ -283,12 +336,12 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
         const double Awr = (double)vr * q_r * invsc;
 #pragma unroll
         for (int l = 0; l < NS; ++l) {
-          const int dl = P->vr[n][l] - P->vf[n][l];
+          const int dl = Kp.vr(n, l) - Kp.vf(n, l);
           if (dl != 0) A[l][m] += (Awf - Awr) * (double)dl;
         }
       }
     }
-  }
+  });
 }
 
 // Production rates (as production_rate without the Jacobian) together with

@@ -121,6 +121,17 @@ int f3d_destroy(f3d_ctx* ctx);
 const char* f3d_last_error(void);
 /* precision of the context's intermediate arrays (sizeof element: 8 or 4) */
 int f3d_real_size(const f3d_ctx* ctx);
+/* Where the context's chemistry kernels take the reaction parameters from
+ * (no counterpart in the reference, whose kernels read reactionProperty):
+ * F3D_KIN_RUNTIME, the parameter block copied from f3d_kinetics (any
+ * mechanism), or the tables compiled in for a mechanism shipped with the
+ * library, used when every kinetics parameter of the context equals the
+ * table's.  Results are the same either way; the tables let the compiler
+ * resolve the stoichiometry. */
+#define F3D_KIN_RUNTIME 0
+#define F3D_KIN_H2_LIDRYER 1
+#define F3D_KIN_AIR5 2
+int f3d_kinetics_source(const f3d_ctx* ctx);
 
 /* ---- kernel-granular entry points (one per @cuda line of solver.jl) ----- */
 /* fluxSplit, split.jl:17-89: Steger-Warming F+/F- at every padded point.
