@@ -350,10 +350,11 @@ __device__ inline void production_rate(const DevParams* __restrict__ P, const do
 // (finite at zero concentration) plus the [M] derivative of three-body and
 // falloff rates with the falloff factor held fixed (the Lindemann part).
 // One pass over the reactions: the transcendentals are shared with wdot.
-template <int NS>
+template <int NS, class K = KinRT>
 __device__ inline void production_rate_block3(const DevParams* __restrict__ P, const double* sc,
                                               double T, const int (&ci)[3], double* wdot,
                                               double (&M)[3][3]) {
+  const K Kp(P);
   const double lgT = f3d_log(T);
   const double invT = 1.0 / T;
   double gi[NS], cc[3];
@@ -371,13 +372,12 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
 #pragma unroll
     for (int i = 0; i < 3; ++i) M[i][j] = 0.0;
   }
-  const int Nr = P->Nr;
-  for (int n = 0; n < Nr; ++n) {
-    const double kf = P->Arr[n][0] * f3d_exp(P->Arr[n][1] * lgT - P->Arr[n][2] * invT);
+  for_reactions(Kp, [&](const int n) {
+    const double kf = Kp.Arr(n, 0) * f3d_exp(Kp.Arr(n, 1) * lgT - Kp.Arr(n, 2) * invT);
     double dg = 0;
 #pragma unroll
-    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(P->vf[n][m] - P->vr[n][m]);
-    const int sg = P->sgm[n];  // kr = kf / Kc, Kc = (RsT)^sgm exp(dg), with 1/RsT formed once
+    for (int m = 0; m < NS; ++m) dg += gi[m] * (double)(Kp.vf(n, m) - Kp.vr(n, m));
+    const int sg = Kp.sgm(n);  // kr = kf / Kc, Kc = (RsT)^sgm exp(dg), with 1/RsT formed once
     const double kr = kf * ((sg > 0 ? ipow(iRsT, sg) : ipow(RsT, -sg)) * f3d_exp(-dg));
     // q_f, q_r: the full mass-action products (same order as production_rate,
     // so wdot is bit-identical to it); o_f, o_r: the products over the
@@ -385,7 +385,7 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     double q_f = kf, q_r = kr, o_f = kf, o_r = kr;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      const int vf = Kp.vf(n, m), vr = Kp.vr(n, m);
       const bool carrier = (m == ci[0]) || (m == ci[1]) || (m == ci[2]);
       if (vf != 0) {
         const double t = ipow(sc[m], vf);
@@ -401,16 +401,16 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     int vfc[3], vrc[3];  // carrier stoichiometry of this reaction
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      vfc[j] = ci[j] >= 0 ? P->vf[n][ci[j]] : 0;
-      vrc[j] = ci[j] >= 0 ? P->vr[n][ci[j]] : 0;
+      vfc[j] = ci[j] >= 0 ? Kp.vf(n, ci[j]) : 0;
+      vrc[j] = ci[j] >= 0 ? Kp.vr(n, ci[j]) : 0;
     }
-    const int rt = P->rtype[n];
+    const int rt = Kp.rtype(n);
     double dlnq_dM = 0.0;
     double fac = 1.0;
     if (rt == 2) {
       double mix = 0.0;
 #pragma unroll
-      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
+      for (int m = 0; m < NS; ++m) mix += Kp.ef(n, m) * sc[m];
       q_f *= mix;
       q_r *= mix;
       dlnq_dM = 1.0 / mix;
@@ -418,13 +418,13 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     } else if (rt == 3) {
       double mix = 0.0;
 #pragma unroll
-      for (int m = 0; m < NS; ++m) mix += P->ef[n][m] * sc[m];
-      const double redP = mix / kf * P->loP[n][0] * f3d_exp(P->loP[n][1] * lgT - P->loP[n][2] * invT);
+      for (int m = 0; m < NS; ++m) mix += Kp.ef(n, m) * sc[m];
+      const double redP = mix / kf * Kp.loP(n, 0) * f3d_exp(Kp.loP(n, 1) * lgT - Kp.loP(n, 2) * invT);
       const double logPred = f3d_log10(redP);
-      const double A_ = P->Troe[n][0];
-      const double logFcent = f3d_log10((1 - A_) * f3d_exp(-T / P->Troe[n][1]) +
-                                        A_ * f3d_exp(-T / P->Troe[n][2]) +
-                                        f3d_exp(-P->Troe[n][3] * invT));
+      const double A_ = Kp.Troe(n, 0);
+      const double logFcent = f3d_log10((1 - A_) * f3d_exp(-T / Kp.Troe(n, 1)) +
+                                        A_ * f3d_exp(-T / Kp.Troe(n, 2)) +
+                                        f3d_exp(-Kp.Troe(n, 3) * invT));
       const double troe_c = -0.4 - 0.67 * logFcent;
       const double troe_n = 0.75 - 1.27 * logFcent;
       const double troe = (troe_c + logPred) / (troe_n - 0.14 * (troe_c + logPred));
@@ -438,7 +438,7 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
     const double dq = q_f - q_r;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
-      const int vf = P->vf[n][m], vr = P->vr[n][m];
+      const int vf = Kp.vf(n, m), vr = Kp.vr(n, m);
       if (vf != vr) wdot[m] += dq * (double)(vr - vf);
     }
 #pragma unroll
@@ -446,7 +446,7 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
       const int cj = ci[j];
       if (cj < 0) continue;
       const int vfj = vfc[j], vrj = vrc[j];
-      double d = (rt != 1) ? dq * dlnq_dM * P->ef[n][cj] : 0.0;
+      double d = (rt != 1) ? dq * dlnq_dM * Kp.ef(n, cj) : 0.0;
       if (vfj != 0 || vrj != 0) {
         // d(k c_j^v_j prod_{m != j} c_m^v_m)/dc_j: the non-carrier product
         // times the other two carriers' powers
@@ -464,11 +464,11 @@ __device__ inline void production_rate_block3(const DevParams* __restrict__ P, c
       for (int i = 0; i < 3; ++i) {
         const int cr = ci[i];
         if (cr < 0) continue;
-        const int dl = P->vr[n][cr] - P->vf[n][cr];
+        const int dl = Kp.vr(n, cr) - Kp.vf(n, cr);
         if (dl != 0) M[i][j] += d * (double)dl;
       }
     }
-  }
+  });
 }
 
 // solve, reactions.jl:463-503 (partial pivoting; row exchanges done with

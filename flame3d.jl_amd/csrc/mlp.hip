@@ -23,6 +23,7 @@
 #include "f3d_chem.h"
 #include "f3d_internal.h"
 #include "f3d_nn.h"
+#include "f3d_mech_tables.h"
 
 namespace f3d {
 
@@ -643,7 +644,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
   if (code != 1) status[id] = (int8_t)code;
 }
 
-template <int NS>
+template <int NS, class K = KinRT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
     k_nn_gate(double* __restrict__ U, double* __restrict__ Q, double* __restrict__ rhoi,
               const DevParams* __restrict__ P, const NNDev* __restrict__ C, Geom g, double dt,
@@ -672,7 +673,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
       for (int n = 0; n < NS; ++n) sc[n] = rn[n] * P->cmw[n];
       if (C->chain_max > 0.0f) {
         const int cidx[3] = {C->chain_idx[0], C->chain_idx[1], C->chain_idx[2]};
-        production_rate_block3<NS>(P, sc, Tn, cidx, wdot, Mc);
+        production_rate_block3<NS, K>(P, sc, Tn, cidx, wdot, Mc);
       } else {
         production_rate<NS, false>(P, sc, Tn, Adum, wdot);
       }
@@ -1220,13 +1221,23 @@ int f3d_nn_chem(f3d_ctx* c, double* U, double* Q, double* rhoi, double dt, int32
   if (c->g.Ns == 9) {
     hipLaunchKernelGGL((k_nn_pred<9>), dim3(nb), dim3(kBlock), 0, s, Q, rhoi, c->dparams,
                        (const NNDev*)c->nn_cfg, c->g, c->nn_out, c->nn_cand, st);
-    hipLaunchKernelGGL((k_nn_gate<9>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
-                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
+    if (c->kin_table == F3D_KIN_H2_LIDRYER)
+      hipLaunchKernelGGL((k_nn_gate<9, KinCT<MechH2LiDryer>>), dim3(nb), dim3(kBlock), 0, s, U, Q,
+                         rhoi, c->dparams, (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand,
+                         c->nn_fbcount, st);
+    else
+      hipLaunchKernelGGL((k_nn_gate<9>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                         (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
   } else {
     hipLaunchKernelGGL((k_nn_pred<5>), dim3(nb), dim3(kBlock), 0, s, Q, rhoi, c->dparams,
                        (const NNDev*)c->nn_cfg, c->g, c->nn_out, c->nn_cand, st);
-    hipLaunchKernelGGL((k_nn_gate<5>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
-                       (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
+    if (c->kin_table == F3D_KIN_AIR5)
+      hipLaunchKernelGGL((k_nn_gate<5, KinCT<MechAir5>>), dim3(nb), dim3(kBlock), 0, s, U, Q,
+                         rhoi, c->dparams, (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand,
+                         c->nn_fbcount, st);
+    else
+      hipLaunchKernelGGL((k_nn_gate<5>), dim3(nb), dim3(kBlock), 0, s, U, Q, rhoi, c->dparams,
+                         (const NNDev*)c->nn_cfg, c->g, dt, c->nn_cand, c->nn_fbcount, st);
   }
   if (timed) (void)hipEventRecord(c->nn_ev[2], s);
   // tile counts -> tile offsets; the chunk sums sit after the tile counts
