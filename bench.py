@@ -984,7 +984,7 @@ def main():
     dom = max(share, key=share.get)
     if dom == "k_nn_chem":
         kc = chem_tab["k_nn_chem"]
-        roofline = {"bound": "mfma", "kernel": "k_nn_chem (FP64 gate epilogue bound)",
+        roofline = {"bound": "mfma", "kernel": "k_nn_net + k_nn_pred + k_nn_gate (FP64 epilogue bound)",
                     "achieved": kc["fp64_TFLOPs"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": kc["fp64_frac"], "traffic": None,
                     "alg_fp64_flop_per_launch": nn_f64, "avg_launch_ms": nn_ms,

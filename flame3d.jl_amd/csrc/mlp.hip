@@ -438,11 +438,12 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
 
 // ---------------------------------------------------------------------------
 // Surrogate chemistry step (pre_input -> network -> post_predict,
-// reactions.jl:2-56) in two kernels: k_nn_net runs the network on the matrix
-// cores, one wave per 64 interior cells, and stores its raw outputs; then
-// k_nn_epi, one thread per cell with the whole register budget for the FP64
-// work, applies the gates and either commits the prediction, with the FP64
-// energy bookkeeping of the backward-Euler integrator, or marks its cell for
+// reactions.jl:2-56) in three kernels: k_nn_net runs the network on the
+// matrix cores, one wave per 64 interior cells, and stores its raw outputs;
+// then the FP64 epilogue, one thread per cell: k_nn_pred (gates 2-4, the
+// predicted state with the FP64 energy bookkeeping of the backward-Euler
+// integrator and its temperature) and k_nn_gate (gates 5-7 on one rate
+// evaluation), which either commits the prediction or marks its cell for
 // the Newton pass and leaves it untouched (status = 1 + gate); the marked
 // cells are then listed in cell order (k_nn_scan, k_nn_compact):
 //   1. Mahalanobis distance of the normalized inputs <= d_max (out of
@@ -461,8 +462,8 @@ __device__ __forceinline__ bool project_weighted(const NNDev* __restrict__ C,
 //   7. chain explosiveness: every eigenvalue of dt d wdot / d c over the chain
 //      carriers below chain_max (near dt lambda = 1 a hot radical-free mixture
 //      has a near-root with a tiny residual while the Newton step ignites it).
-// Between the two kernels the outputs take 4 B per cell and output
-// ([Ns][cells], 0.6 GB at 256^3 for Ns = 9, written and read once).
+// Between the network and the epilogue the outputs take 4 B per cell and
+// output ([Ns][cells], 0.6 GB at 256^3 for Ns = 9, written and read once).
 // ---------------------------------------------------------------------------
 template <int NS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
