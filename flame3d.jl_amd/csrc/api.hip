@@ -103,6 +103,7 @@ static int fill_params(DevParams& P, const f3d_dims* d, const f3d_thermo* th,
   for (int n = 0; n < d->Ns; ++n) {
     P.rmw[n] = 1.0 / P.mw[n];
     P.Rmw[n] = P.Ru / P.mw[n];
+    P.Rmw_f[n] = (float)P.Rmw[n];
     P.cmw[n] = 1e-6 / P.mw[n];
     P.hfm[n] = P.lo[n][5] * P.Ru / P.mw[n];
     for (int l = 0; l < d->Ns; ++l) P.wnl[n][l] = P.mw[n] / P.mw[l];

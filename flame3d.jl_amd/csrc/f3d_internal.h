@@ -41,6 +41,7 @@ struct DevParams {
   float mw_f[MAXNS], sep_f[MAXNS], lo_f[MAXNS][7], hi_f[MAXNS][7];
   float visc_f[MAXNS][5], cond_f[MAXNS][5], bdiff_f[MAXNS][MAXNS][5];
   float Ru_f, WENOeps_f, phi1_f, phi2_f, SWeps2_f, UP7_f[7];
+  float Rmw_f[MAXNS];  // Ru / W_n rounded to float
   // Wilke's species-pair constants for the FP32 mixture rule (W_l / W_n)^(1/4),
   // 1 / sqrt(8 (1 + W_n / W_l)) and W_n / W_l
   float wr4_f[MAXNS][MAXNS], c8_f[MAXNS][MAXNS], iwr_f[MAXNS][MAXNS];
@@ -144,6 +145,7 @@ struct PT<double> {
   __device__ static const double* cond(const DevParams* P, int n) { return P->cond[n]; }
   __device__ static const double* bdiff(const DevParams* P, int n, int nn) { return P->bdiff[n][nn]; }
   __device__ static double Ru(const DevParams* P) { return P->Ru; }
+  __device__ static double Rmw(const DevParams* P, int n) { return P->Rmw[n]; }
   __device__ static double weps(const DevParams* P) { return P->WENOeps; }
   __device__ static double phi1(const DevParams* P) { return P->phi1; }
   __device__ static double phi2(const DevParams* P) { return P->phi2; }
@@ -161,6 +163,7 @@ struct PT<float> {
   __device__ static const float* cond(const DevParams* P, int n) { return P->cond_f[n]; }
   __device__ static const float* bdiff(const DevParams* P, int n, int nn) { return P->bdiff_f[n][nn]; }
   __device__ static float Ru(const DevParams* P) { return P->Ru_f; }
+  __device__ static float Rmw(const DevParams* P, int n) { return P->Rmw_f[n]; }
   __device__ static float weps(const DevParams* P) { return P->WENOeps_f; }
   __device__ static float phi1(const DevParams* P) { return P->phi1_f; }
   __device__ static float phi2(const DevParams* P) { return P->phi2_f; }
@@ -189,7 +192,7 @@ __device__ inline void h_specs(const DevParams* P, R T, R* h) {
     } else {
       v = hc[0] * T + hc[1] * T2 + hc[2] * T3 + hc[3] * T4 + hc[4] * T5 + (hc[5] - lo[5]);
     }
-    h[n] = v * (PT<R>::Ru(P) / PT<R>::mw(P, n));
+    h[n] = v * PT<R>::Rmw(P, n);  // Ru / W_n (the same quotient, rounded once on the host)
   }
 }
 

@@ -8,18 +8,30 @@
 
 namespace f3d {
 
+// a / b: IEEE division in FP64; in FP32 (the float flux path, whose
+// divisors here are normal numbers: sound speeds, pressures, metric norms and
+// WENO indicators floored by eps = 1e-26) the hardware reciprocal times a,
+// within 1 ulp, instead of the division's denormal-safe expansion
+template <typename R>
+__device__ __forceinline__ R qdiv(R a, R b) {
+  if constexpr (sizeof(R) == 4)
+    return a * __builtin_amdgcn_rcpf(b);
+  else
+    return a / b;
+}
+
 // fluxSplit at one point, split.jl:17-89: F+ and F- of the five conserved
 // fluxes from rho, u, v, w, p, e and the sweep's three metrics.
 template <typename R>
 __device__ inline void split_point(R rho, R u, R v, R w, R pr, R ei, R A1, R A2, R A3, R SWeps2,
                                    R* __restrict__ Fp, R* __restrict__ Fm) {
-  const R gam = pr / ei + R(1);
-  const R c = sqrt(gam * pr / rho);
+  const R gam = qdiv(pr, ei) + R(1);
+  const R c = sqrt(qdiv(gam * pr, rho));
   R ss = sqrt(A1 * A1 + A2 * A2 + A3 * A3);
   const R E1 = A1 * u + A2 * v + A3 * w;
   const R E2 = E1 - c * ss;
   const R E3 = E1 + c * ss;
-  ss = R(1) / ss;
+  ss = qdiv(R(1), ss);
   A1 *= ss;
   A2 *= ss;
   A3 *= ss;
@@ -33,8 +45,8 @@ __device__ inline void split_point(R rho, R u, R v, R w, R pr, R ei, R A1, R A2,
   const R vvc1 = (uc1 * uc1 + vc1 * vc1 + wc1 * wc1) * R(0.5);
   const R vvc2 = (uc2 * uc2 + vc2 * vc2 + wc2 * wc2) * R(0.5);
   const R vv = (gam - R(1)) * (u * u + v * v + w * w);
-  const R W2 = (R(3) - gam) / (R(2) * (gam - R(1))) * (c * c);
-  const R tmp1 = rho / (R(2) * gam);
+  const R W2 = qdiv(R(3) - gam, R(2) * (gam - R(1))) * (c * c);
+  const R tmp1 = qdiv(rho, R(2) * gam);
   const R tmp2 = R(2) * (gam - R(1));
   Fp[0] = tmp1 * (tmp2 * E1P + E2P + E3P);
   Fp[1] = tmp1 * (tmp2 * E1P * u + E2P * uc1 + E3P * uc2);
@@ -68,10 +80,10 @@ __device__ inline void zweights(R s11, R s22, R s33, R e, R& w1, R& w2, R& w3) {
   } else {
     const R q1 = e + s11, q2 = e + s22, q3 = e + s33;
     const R qm = fmin(q1, fmin(q2, q3));
-    const R T = tau / qm;
-    const R u = R(1) / (R(1) + T * T);
+    const R T = qdiv(tau, qm);
+    const R u = qdiv(R(1), R(1) + T * T);
     const R v = R(1) - u;
-    const R a = qm / q1, b = qm / q2, c = qm / q3;
+    const R a = qdiv(qm, q1), b = qdiv(qm, q2), c = qdiv(qm, q3);
     w1 = (u + v * (a * a)) * R(1);
     w2 = (u + v * (b * b)) * R(6);
     w3 = (u + v * (c * c)) * R(3);
@@ -118,10 +130,10 @@ __device__ inline R weno7z(const R* v, R eps) {
   } else {
     const R q0 = eps + b0, q1 = eps + b1, q2 = eps + b2, q3 = eps + b3;
     const R qm = fmin(fmin(q0, q1), fmin(q2, q3));
-    const R T = tau / qm;
-    const R u = R(1) / (R(1) + T * T);
+    const R T = qdiv(tau, qm);
+    const R u = qdiv(R(1), R(1) + T * T);
     const R vv = R(1) - u;
-    const R s0 = qm / q0, s1 = qm / q1, s2 = qm / q2, s3 = qm / q3;
+    const R s0 = qdiv(qm, q0), s1 = qdiv(qm, q1), s2 = qdiv(qm, q2), s3 = qdiv(qm, q3);
     w0 = (u + vv * (s0 * s0)) * R(1);
     w1 = (u + vv * (s1 * s1)) * R(12);
     w2 = (u + vv * (s2 * s2)) * R(18);
@@ -131,7 +143,7 @@ __device__ inline R weno7z(const R* v, R eps) {
   const R c1 = b - R(5) * c + R(13) * d + R(3) * e;
   const R c2 = -c + R(7) * d + R(7) * e - f;
   const R c3 = R(3) * d + R(13) * e - R(5) * f + g;
-  return (w0 * c0 + w1 * c1 + w2 * c2 + w3 * c3) / (R(12) * (w0 + w1 + w2 + w3));
+  return qdiv(w0 * c0 + w1 * c1 + w2 * c2 + w3 * c3, R(12) * (w0 + w1 + w2 + w3));
 }
 
 template <typename R>
@@ -176,7 +188,7 @@ __device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams*
     t2 = R(3) * V3 - R(4) * V4 + V5;
     R s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
     zweights<R>(s11, s22, s33, e, s11, s22, s33);
-    R invsum = R(1) / (s11 + s22 + s33);
+    R invsum = qdiv(R(1), s11 + s22 + s33);
     R v1 = R(2) * V1 - R(7) * V2 + R(11) * V3;
     R v2 = -V2 + R(5) * V3 + R(2) * V4;
     R v3 = R(2) * V3 + R(5) * V4 - V5;
@@ -197,7 +209,7 @@ __device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams*
     t2 = R(3) * V3 - R(4) * V2 + V1;
     s33 = tmp1 * (t * t) + R(0.25) * (t2 * t2);
     zweights<R>(s11, s22, s33, e, s11, s22, s33);
-    invsum = R(1) / (s11 + s22 + s33);
+    invsum = qdiv(R(1), s11 + s22 + s33);
     v1 = R(11) * V3 - R(7) * V4 + R(2) * V5;
     v2 = -V4 + R(5) * V3 + R(2) * V2;
     v3 = R(2) * V3 + R(5) * V2 - V1;

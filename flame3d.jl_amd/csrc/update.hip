@@ -3,6 +3,7 @@
 // on-device boundary fills that replace boundary.jl's case fills and the MPI
 // halo exchange on a single GPU.
 #include "f3d_internal.h"
+#include "f3d_stencil.h"
 
 namespace f3d {
 
@@ -104,9 +105,9 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
     const R* p = PT<R>::cond(P, n);
     const R lk = sqT * (p[0] + lgT * p[1] + lgT2 * p[2] + lgT3 * p[3] + lgT4 * p[4]);
     s1 += X[n] * lk;
-    s2 += X[n] / lk;
+    s2 += qdiv(X[n], lk);
   }
-  lambda = R(0.5) * (s1 + R(1) / s2);
+  lambda = R(0.5) * (s1 + qdiv(R(1), s2));
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
     const R* p = PT<R>::visc(P, n);
@@ -121,7 +122,7 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
     // factors precomputed, so no division or square root per pair
     R isq[NS];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) isq[n] = R(1) / sqi[n];
+    for (int n = 0; n < NS; ++n) isq[n] = qdiv(R(1), sqi[n]);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
 #pragma unroll
@@ -158,7 +159,7 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
     R tmp = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) tmp += X[l] * phi[n][l];
-    m += X[n] * mui[n] / tmp;
+    m += qdiv(X[n] * mui[n], tmp);
   }
   mu = m;
   // binary diffusion, D_kl P = T^1.5 poly (thermo.jl:349-367)
@@ -169,8 +170,10 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
     for (int nn = n; nn < NS; ++nn) {
       const R* p = PT<R>::bdiff(P, n, nn);
       const R tmp = T15 * (p[0] + lgT * p[1] + lgT2 * p[2] + lgT3 * p[3] + lgT4 * p[4]);
-      phi[nn][n] = tmp;
-      phi[n][nn] = tmp;
+      // FP32: the pair's reciprocal once (the sums below multiply by it)
+      const R v = (sizeof(R) == 4) ? qdiv(R(1), tmp) : tmp;
+      phi[nn][n] = v;
+      phi[n][nn] = v;
     }
   }
   // D_k = (1 - X_k) / (P sum_{l!=k} X_l / D_kl + eps) with 1 - X_k taken as
@@ -183,11 +186,11 @@ __device__ inline void mixture_props(const DevParams* __restrict__ P, R T, R Pr,
 #pragma unroll
     for (int nn = 0; nn < NS; ++nn) {
       if (nn == n) continue;
-      s += X[nn] / phi[n][nn];
+      s += (sizeof(R) == 4) ? X[nn] * phi[n][nn] : X[nn] / phi[n][nn];
       sx += X[nn];
     }
     s *= Pr;
-    Diff[n] = sx / (s + PT<R>::eps());
+    Diff[n] = qdiv(sx, s + PT<R>::eps());
   }
 }
 
