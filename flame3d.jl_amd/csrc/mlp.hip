@@ -40,7 +40,9 @@ __device__ inline bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const b
 __device__ inline float gelu(float x, float& dg) {
   const float c = 0.7978845608028654f, k = 0.044715f;
   const float u = c * (x + k * x * x * x);
-  const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));  // tanh(u), saturates cleanly
+  // tanh(u) = 1 - 2 / (1 + e^2u), with the hardware reciprocal (1 ulp; the
+  // activations are rounded to bf16 next): saturates cleanly at +-1
+  const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
   dg = 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * k * x * x);
   return 0.5f * x * (1.0f + t);
 }
@@ -494,16 +496,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
       // pre_input (reactions.jl:2-23): T, log P, Box-Cox(Y), z-score
       const double rinv = 1.0 / fmax(Q[p], kEps);
       const float lam = C->boxcox_lambda;
+      const float ilam = __builtin_amdgcn_rcpf(lam);  // the inputs are rounded to bf16 next
       float x[NIN];
       x[0] = (float)T;
       x[1] = logf((float)Q[p + 4 * NP]);
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
         const float Y = (float)fmax(rhoi[p + n * NP] * rinv, (double)C->y_floor);
-        x[2 + n] = (powf(Y, lam) - 1.0f) / lam;
+        x[2 + n] = (powf(Y, lam) - 1.0f) * ilam;
       }
 #pragma unroll
-      for (int k = 0; k < NIN; ++k) xn[k] = (x[k] - C->in_mean[k]) / C->in_std[k];
+      for (int k = 0; k < NIN; ++k)
+        xn[k] = (x[k] - C->in_mean[k]) * __builtin_amdgcn_rcpf(C->in_std[k]);
     }
   }
   // a tile with no reacting cell (cold gas, tagged cells) skips the network;
