@@ -162,8 +162,11 @@ __device__ inline int weno_branch(R phif, const DevParams* __restrict__ P) {
 // WENO_x / WENO_y / WENO_z face value, schemes.jl:86-437, on the eight
 // values v[k] = f[ic + (k - 4) s] of Fp (p) and Fm (m) around the face i-1/2
 // (ic its right cell, s the sweep stride).
-template <typename R>
+// BR: the branch as a compile-time constant (-1: read br at run time), so
+// a caller evaluating several variables at one face tests the branch once.
+template <typename R, int BR = -1>
 __device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams* __restrict__ P) {
+  if (BR >= 0) br = BR;
   const R* c = PT<R>::up7(P);
   if (br == 0) {
     const R fpv = c[0] * p[0] + c[1] * p[1] + c[2] * p[2] + c[3] * p[3] + c[4] * p[4] +
@@ -219,6 +222,34 @@ __device__ inline R weno_face_v(const R* p, const R* m, int br, const DevParams*
     const R fpv = p[3] + R(0.5) * minmod(p[4] - p[3], p[3] - p[2]);
     const R fmv = m[4] - R(0.5) * minmod(m[5] - m[4], m[4] - m[3]);
     return fpv + fmv;
+  }
+}
+
+// the hybrid face values of NV variables at one face: the branch is
+// tested once for all of them (get(n, pv, mv) gathers variable n's stencil)
+template <typename R, int NV, class G>
+__device__ inline void weno_faces_v(int br, const DevParams* __restrict__ P, G&& get, R* out) {
+  if (br == 0) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      R pv[8], mv[8];
+      get(n, pv, mv);
+      out[n] = weno_face_v<R, 0>(pv, mv, 0, P);
+    }
+  } else if (br == 1) {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      R pv[8], mv[8];
+      get(n, pv, mv);
+      out[n] = weno_face_v<R, 1>(pv, mv, 1, P);
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      R pv[8], mv[8];
+      get(n, pv, mv);
+      out[n] = weno_face_v<R, 2>(pv, mv, 2, P);
+    }
   }
 }
 

@@ -92,16 +92,13 @@ __global__ void __launch_bounds__(kXSeg)
   float fv[NV];
   if (t < nc) {
     const int br = weno_branch<float>(fmaxf(sphi[t + 3], sphi[t + 4]), P);
-#pragma unroll
-    for (int n = 0; n < NV; ++n) {
-      float pv[8], mv[8];
+    weno_faces_v<float, NV>(br, P, [&](int n, float* pv, float* mv) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         pv[k] = sp[n][t + k];
         mv[k] = sm[n][t + k];
       }
-      fv[n] = weno_face_v<float>(pv, mv, br, P);
-    }
+    }, fv);
   }
   // the segment's last face, one variable per thread
   float fl = 0.0f;
@@ -237,8 +234,13 @@ __device__ inline void conv_march_flow(float* __restrict__ rhs, const double* __
     const float phB = in[0].phi;
     const int br = weno_branch<float>(fmaxf(phA, phB), P);
     float cur[5];
+    weno_faces_v<float, 5>(br, P, [&](int n, float* pv, float* mv) {
 #pragma unroll
-    for (int n = 0; n < 5; ++n) cur[n] = weno_face_v<float>(wp[n], wm[n], br, P);
+      for (int k = 0; k < 8; ++k) {
+        pv[k] = wp[n][k];
+        mv[k] = wm[n][k];
+      }
+    }, cur);
     if (f > m.c0) {
       const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
@@ -335,9 +337,16 @@ __device__ inline void conv_march_species(float* __restrict__ rhs, const double*
     const float phB = in[0].phi;
     const int br = weno_branch<float>(fmaxf(phA, phB), P);
     float cur[3];
+    weno_faces_v<float, 3>(br, P, [&](int kk, float* pv, float* mv) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        pv[k] = wp[kk][k];
+        mv[k] = wm[kk][k];
+      }
+    }, cur);
 #pragma unroll
     for (int kk = 0; kk < 3; ++kk)
-      cur[kk] = (k0 + kk < NS) ? weno_face_v<float>(wp[kk], wm[kk], br, P) : 0.0f;
+      if (k0 + kk >= NS) cur[kk] = 0.0f;
     if (f > m.c0) {
       const long cell = m.cell0 + (long)(f - 1) * m.cstr;
 #pragma unroll
@@ -364,7 +373,7 @@ __device__ inline void conv_march_species(float* __restrict__ rhs, const double*
 }
 
 template <int NS>
-__global__ void __launch_bounds__(64 * (1 + (NS + 2) / 3))
+__global__ void __launch_bounds__(64 * (1 + (NS + 2) / 3)) __attribute__((amdgpu_waves_per_eu(3)))
     k_sw_conv_m(float* __restrict__ rhs, const double* __restrict__ Q,
                 const double* __restrict__ rhoi, const float* __restrict__ A1,
                 const float* __restrict__ A2, const float* __restrict__ A3,
