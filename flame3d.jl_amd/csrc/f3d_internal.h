@@ -64,6 +64,7 @@ struct f3d_ctx {
   int sensor_kind = 0;  // F3D_SENSOR_PRESSURE / F3D_SENSOR_DUCROS
   int stage_kind = 0;   // F3D_STAGE_FUSED / F3D_STAGE_GRANULAR (F3D_FP32 contexts)
   int kin_table = 0;    // F3D_KIN_*: the chemistry kernels' reaction-parameter source
+  long spec_waves = 0;  // species-diffusion march waves resident on the device (0: not queried)
   f3d::DevParams host_params;
   f3d::DevParams* dparams = nullptr;
   // boundary fills

@@ -31,7 +31,7 @@ namespace f3d {
 
 constexpr int kXSeg = 256;  // cells per x-line block
 constexpr int kConvSeg = 32;  // cells per march segment (convective)
-constexpr int kDiffSeg = 64;  // cells per march segment (diffusive)
+constexpr int kViscSeg = 64;  // cells per march segment (viscous)
 
 __device__ inline void rhs_put(float* __restrict__ r, long i, float d, int acc) {
   r[i] = acc ? r[i] + d : d;
@@ -863,6 +863,38 @@ static void conv_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const doub
   }
 }
 
+// Segments per line of the species-diffusion march.  Its registers hold few
+// waves per SIMD (3 for Ns = 9), so the segment count is chosen for whole
+// rounds of resident waves: cost ~ rounds x (segment length + 4 priming
+// rows), e.g. 3 segments of a 256-cell line at 256^3 (3072 waves, one round
+// on 256 CUs) instead of 4 (4096 waves, a second round of 1024).
+template <int NS>
+static int spec_segments(f3d_ctx* c, int nd, long lines) {
+  if (c->spec_waves == 0) {
+    int nb = 0, ncu = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void*>(&k_sw_spec_m<NS>), 64, 0) != hipSuccess || nb < 1)
+      nb = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+      ncu = 1;
+    c->spec_waves = (long)nb * ncu;
+  }
+  const long cap = c->spec_waves;
+  int best = 1;
+  long bc = -1;
+  for (int n = 1; n <= max(1, nd / 16); ++n) {
+    const long rounds = (lines * n + cap - 1) / cap;
+    const long cost = rounds * ((nd + n - 1) / n + 4);
+    if (bc < 0 || cost < bc) {
+      bc = cost;
+      best = n;
+    }
+  }
+  return best;
+}
+
 template <int NS>
 static void diff_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const float* Yi,
                     const FMet& M, const float* J, const float* lam, const float* mu,
@@ -877,12 +909,13 @@ static void diff_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const floa
                        M, J, lam, mu, D, c->dparams, g);
   } else {
     const int nd = dir == 1 ? g.Ny : g.Nz;
-    const int nseg = (nd + kDiffSeg - 1) / kDiffSeg;
-    const int seg = (nd + nseg - 1) / nseg;
-    const dim3 grid((g.Nx + 63) / 64, dir == 1 ? g.Nz : g.Ny, nseg);
-    hipLaunchKernelGGL(k_sw_visc_m, grid, dim3(64), 0, s, rhs, Q, M, J, lam, mu, g, dir, seg);
-    hipLaunchKernelGGL((k_sw_spec_m<NS>), grid, dim3(64), 0, s, rhs, Q, Yi, M, J, D, c->dparams,
-                       g, dir, seg);
+    const int nsv = (nd + kViscSeg - 1) / kViscSeg, sv = (nd + nsv - 1) / nsv;
+    const unsigned nl = (g.Nx + 63) / 64, nt = dir == 1 ? g.Nz : g.Ny;
+    const int nss = spec_segments<NS>(c, nd, (long)nl * nt), ss = (nd + nss - 1) / nss;
+    hipLaunchKernelGGL(k_sw_visc_m, dim3(nl, nt, nsv), dim3(64), 0, s, rhs, Q, M, J, lam, mu, g,
+                       dir, sv);
+    hipLaunchKernelGGL((k_sw_spec_m<NS>), dim3(nl, nt, nss), dim3(64), 0, s, rhs, Q, Yi, M, J, D,
+                       c->dparams, g, dir, ss);
   }
 }
 
