@@ -213,9 +213,8 @@ def kernel_symbols(name, Ns):
         "mixture": [(f"void f3d::k_mixture<{Ns}, float>", 1.0)],
         "sweep_conv": [(f"void f3d::k_sw_conv_x<{Ns}>", 1 / 3),
                        (f"void f3d::k_sw_conv_m<{Ns}>", 2 / 3)],
-        "sweep_diff": [(f"void f3d::k_sw_diff_x<0, {Ns}>", 1 / 3),
-                       (f"void f3d::k_sw_diff_x<1, {Ns}>", 1 / 3),
-                       ("void f3d::k_sw_visc_m", 2 / 3), (f"void f3d::k_sw_spec_m<{Ns}>", 2 / 3)],
+        "sweep_diff": [("f3d::k_sw_visc_x", 1 / 3), (f"void f3d::k_sw_spec_x<{Ns}>", 1 / 3),
+                       ("f3d::k_sw_visc_m", 2 / 3), (f"void f3d::k_sw_spec_m<{Ns}>", 2 / 3)],
         "stage_end": [(f"void f3d::k_sw_stage_end<{Ns}>", 1.0)],
     }.get(name)
 

@@ -398,60 +398,6 @@ struct FMet {
   const float* m[9];
 };
 
-template <int NS>
-__device__ inline void spec_face(const double* __restrict__ Q, const float* __restrict__ Yi,
-                                 const FMet& M, const float* __restrict__ J,
-                                 const float* __restrict__ D, const DevParams* __restrict__ P,
-                                 const Geom& g, long ic, int dir, float* out) {
-  const long st[3] = {1, (long)g.Nxt, (long)g.Nxt * g.Nyt};
-  const long s = st[dir];
-  const long im = ic - s;
-  const long NP = g.NP;
-  float m[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = (M.m[q][im] + M.m[q][ic]) * 0.5f;
-  const float Jac = (J[im] + J[ic]) * 0.5f;
-  const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
-  const float c12 = 1.0f / 12.0f;
-  const float rho = ((float)Q[im] + (float)Q[ic]) * 0.5f;
-  const float T = ((float)Q[im + 5 * NP] + (float)Q[ic + 5 * NP]) * 0.5f;
-  float hi[NS], Vk1[NS], Vk2[NS], Vk3[NS];
-  h_specs<NS, float>(P, T, hi);
-  float sum1 = 0, sum2 = 0, sum3 = 0;
-#pragma unroll
-  for (int n = 0; n < NS; ++n) {
-    const float* y = Yi + (long)n * NP;
-    const float rhoDi = (D[im + (long)n * NP] + D[ic + (long)n * NP]) * rho;
-    float dY[3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t)
-      dY[t] = (t == dir) ? 1.25f * (y[ic] - y[im]) - c12 * (y[ic + s] - y[ic - 2 * s])
-                         : 0.25f * (y[ic + st[t]] + y[im + st[t]] - y[ic - st[t]] - y[im - st[t]]);
-    const float Vx = (dY[0] * m[0] + dY[1] * m[3] + dY[2] * m[6]) * Jac * rhoDi;
-    const float Vy = (dY[0] * m[1] + dY[1] * m[4] + dY[2] * m[7]) * Jac * rhoDi;
-    const float Vz = (dY[0] * m[2] + dY[1] * m[5] + dY[2] * m[8]) * Jac * rhoDi;
-    Vk1[n] = Vx;
-    Vk2[n] = Vy;
-    Vk3[n] = Vz;
-    sum1 += Vx;
-    sum2 += Vy;
-    sum3 += Vz;
-  }
-  float fh1 = 0, fh2 = 0, fh3 = 0;
-#pragma unroll
-  for (int n = 0; n < NS; ++n) {
-    const float Yn = 0.5f * (Yi[im + (long)n * NP] + Yi[ic + (long)n * NP]);
-    const float V1 = Vk1[n] - sum1 * Yn;
-    const float V2 = Vk2[n] - sum2 * Yn;
-    const float V3 = Vk3[n] - sum3 * Yn;
-    out[n] = V1 * n0 + V2 * n1 + V3 * n2;
-    fh1 += V1 * hi[n];
-    fh2 += V2 * hi[n];
-    fh3 += V3 * hi[n];
-  }
-  out[NS] = n0 * fh1 + n1 * fh2 + n2 * fh3;
-}
-
 __device__ inline void visc_face(const double* __restrict__ Q, const FMet& M,
                                  const float* __restrict__ J, const float* __restrict__ lam,
                                  const float* __restrict__ mu, const Geom& g, long ic, int dir,
@@ -512,16 +458,13 @@ __device__ inline void visc_face(const double* __restrict__ Q, const FMet& M,
   out[3] = n0 * E1 + n1 * E2 + n2 * E3;
 }
 
-// x-line block of the diffusive sweeps: face values in LDS, then the
-// difference of each cell's two faces.  KIND 0: viscous (rhs 1..4), KIND 1:
-// species diffusion (rhs 5.. and the enthalpy flux into rhs 4).
-template <int KIND, int NS>
+// x-line block of the viscous sweep: face values in LDS, then the
+// difference of each cell's two faces (rhs 1..4).
 __global__ void __launch_bounds__(kXSeg)
-    k_sw_diff_x(float* __restrict__ rhs, const double* __restrict__ Q,
-                const float* __restrict__ Yi, FMet M, const float* __restrict__ J,
-                const float* __restrict__ lam, const float* __restrict__ mu,
-                const float* __restrict__ D, const DevParams* __restrict__ P, Geom g) {
-  constexpr int NO = KIND == 0 ? 4 : NS + 1;
+    k_sw_visc_x(float* __restrict__ rhs, const double* __restrict__ Q, FMet M,
+                const float* __restrict__ J, const float* __restrict__ lam,
+                const float* __restrict__ mu, Geom g) {
+  constexpr int NO = 4;
   __shared__ float sf[NO][kXSeg + 1];
   const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
   const unsigned L = xcd_block();
@@ -534,10 +477,7 @@ __global__ void __launch_bounds__(kXSeg)
   const long row = (long)(c0 + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
   auto face = [&](int f) {
     float out[NO];
-    if (KIND == 0)
-      visc_face(Q, M, J, lam, mu, g, row + f, 0, out);
-    else
-      spec_face<NS>(Q, Yi, M, J, D, P, g, row + f, 0, out);
+    visc_face(Q, M, J, lam, mu, g, row + f, 0, out);
 #pragma unroll
     for (int n = 0; n < NO; ++n) sf[n][f] = out[n];
   };
@@ -548,15 +488,142 @@ __global__ void __launch_bounds__(kXSeg)
     const long NC = (long)g.Nx * g.Ny * g.Nz;
     const long cell = (c0 + t) + (long)g.Nx * (b + (long)g.Ny * c);
     // diffusive fluxes enter with the opposite sign of the convective ones
-    if (KIND == 0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) rhs_put(rhs, (q + 1) * NC + cell, sf[q][t + 1] - sf[q][t], 1);
-    } else {
+    for (int q = 0; q < 4; ++q) rhs_put(rhs, (q + 1) * NC + cell, sf[q][t + 1] - sf[q][t], 1);
+  }
+}
+
+// x-line block of the species-diffusion sweep with each cell's inputs read
+// once: the metrics, J, rho and T of the cells -1 .. nc are staged in LDS,
+// then species by species Y_k of the cells -2 .. nc+1 and the transverse
+// differences and D_k of the cells -1 .. nc (two buffers, one barrier per
+// species); face f (between cells f-1 and f) forms its gradient from them
+// (viscous.jl:430-507: the face average of the cells' transverse central
+// differences, the fourth-order normal difference).  A full segment has
+// nc + 1 = kXSeg + 1 faces: its last thread also takes the last face.
+template <int NS>
+__global__ void __launch_bounds__(kXSeg)
+    k_sw_spec_x(float* __restrict__ rhs, const double* __restrict__ Q,
+                const float* __restrict__ Yi, FMet M, const float* __restrict__ J,
+                const float* __restrict__ D, const DevParams* __restrict__ P, Geom g) {
+  constexpr int NE = kXSeg + 2;  // cells -1 .. nc
+  constexpr int NY = kXSeg + 4;  // cells -2 .. nc + 1
+  __shared__ float sm[12][NE];   // metrics, J, rho, T
+  __shared__ float sy[2][NY], scy[2][NE], scz[2][NE], sd[2][NE];
+  __shared__ float sf[NS + 1][kXSeg + 1];
+  __shared__ float sx[4][NS];  // the extra face's V_x, V_y, V_z and Y
+  const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
+  const unsigned L = xcd_block();
+  const int seg = (int)(L % (unsigned)nseg);
+  const int line = (int)(L / (unsigned)nseg);
+  const int b = line % g.Ny, c = line / g.Ny;
+  const int c0 = seg * kXSeg;
+  const int nc = min(kXSeg, g.Nx - c0);
+  const int t = threadIdx.x;
+  const long NP = g.NP, ty = g.Nxt, tz = (long)g.Nxt * g.Nyt;
+  const long row = (long)(c0 + NG) + ty * (b + NG) + tz * (c + NG);  // cell 0
+  for (int i = t; i < nc + 2; i += kXSeg) {
+    const long ic = row + i - 1;
 #pragma unroll
-      for (int n = 0; n < NS; ++n)
-        rhs_put(rhs, (5 + n) * NC + cell, sf[n][t + 1] - sf[n][t], 1);
-      rhs_put(rhs, 4 * NC + cell, sf[NS][t + 1] - sf[NS][t], 1);
+    for (int q = 0; q < 9; ++q) sm[q][i] = M.m[q][ic];
+    sm[9][i] = J[ic];
+    sm[10][i] = (float)Q[ic];
+    sm[11][i] = (float)Q[ic + 5 * NP];
+  }
+  const bool own = t <= nc;                       // face t
+  const bool extra = nc == kXSeg && t == kXSeg - 1;  // and face kXSeg
+  const float c12 = 1.0f / 12.0f;
+  float V[3][NS], Yn[NS];
+  // V_k of face f (staged index f: cell f-1, f + 1: cell f) for species n
+  auto fspec = [&](int f, int bb, float* out) {
+    float m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = (sm[q][f] + sm[q][f + 1]) * 0.5f;
+    const float Jac = (sm[9][f] + sm[9][f + 1]) * 0.5f;
+    const float rho = (sm[10][f] + sm[10][f + 1]) * 0.5f;
+    const float rhoDi = (sd[bb][f] + sd[bb][f + 1]) * rho;
+    const float dY0 = 1.25f * (sy[bb][f + 2] - sy[bb][f + 1]) - c12 * (sy[bb][f + 3] - sy[bb][f]);
+    const float dY1 = 0.25f * (scy[bb][f + 1] + scy[bb][f]);
+    const float dY2 = 0.25f * (scz[bb][f + 1] + scz[bb][f]);
+    out[0] = (dY0 * m[0] + dY1 * m[3] + dY2 * m[6]) * Jac * rhoDi;
+    out[1] = (dY0 * m[1] + dY1 * m[4] + dY2 * m[7]) * Jac * rhoDi;
+    out[2] = (dY0 * m[2] + dY1 * m[5] + dY2 * m[8]) * Jac * rhoDi;
+    out[3] = 0.5f * (sy[bb][f + 1] + sy[bb][f + 2]);
+  };
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const int bb = n & 1;
+    const float* y = Yi + (long)n * NP;
+    for (int i = t; i < nc + 4; i += kXSeg) sy[bb][i] = y[row + i - 2];
+    for (int i = t; i < nc + 2; i += kXSeg) {
+      const long ic = row + i - 1;
+      scy[bb][i] = y[ic + ty] - y[ic - ty];
+      scz[bb][i] = y[ic + tz] - y[ic - tz];
+      sd[bb][i] = D[ic + (long)n * NP];
     }
+    __syncthreads();
+    float o[4];
+    if (own) {
+      fspec(t, bb, o);
+      V[0][n] = o[0];
+      V[1][n] = o[1];
+      V[2][n] = o[2];
+      Yn[n] = o[3];
+    }
+    if (extra) {
+      fspec(kXSeg, bb, o);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sx[k][n] = o[k];
+    }
+  }
+  // the face's corrected diffusion velocities, its species fluxes and the
+  // enthalpy flux n . sum_k V_k h_k
+  auto ffin = [&](int f, const float (&Vf)[3][NS], const float (&Yf)[NS]) {
+    const float n0 = (sm[0][f] + sm[0][f + 1]) * 0.5f;
+    const float n1 = (sm[1][f] + sm[1][f + 1]) * 0.5f;
+    const float n2 = (sm[2][f] + sm[2][f + 1]) * 0.5f;
+    const float T = (sm[11][f] + sm[11][f + 1]) * 0.5f;
+    float hi[NS];
+    h_specs<NS, float>(P, T, hi);
+    float s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      s1 += Vf[0][n];
+      s2 += Vf[1][n];
+      s3 += Vf[2][n];
+    }
+    float fh1 = 0, fh2 = 0, fh3 = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const float V1 = Vf[0][n] - s1 * Yf[n];
+      const float V2 = Vf[1][n] - s2 * Yf[n];
+      const float V3 = Vf[2][n] - s3 * Yf[n];
+      sf[n][f] = V1 * n0 + V2 * n1 + V3 * n2;
+      fh1 += V1 * hi[n];
+      fh2 += V2 * hi[n];
+      fh3 += V3 * hi[n];
+    }
+    sf[NS][f] = n0 * fh1 + n1 * fh2 + n2 * fh3;
+  };
+  if (own) ffin(t, V, Yn);
+  if (extra) {
+    float Vx[3][NS], Yx[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      Vx[0][n] = sx[0][n];
+      Vx[1][n] = sx[1][n];
+      Vx[2][n] = sx[2][n];
+      Yx[n] = sx[3][n];
+    }
+    ffin(kXSeg, Vx, Yx);
+  }
+  __syncthreads();
+  if (t < nc) {
+    const long NC = (long)g.Nx * g.Ny * g.Nz;
+    const long cell = (c0 + t) + (long)g.Nx * (b + (long)g.Ny * c);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) rhs_put(rhs, (5 + n) * NC + cell, sf[n][t + 1] - sf[n][t], 1);
+    rhs_put(rhs, 4 * NC + cell, sf[NS][t + 1] - sf[NS][t], 1);
   }
 }
 
@@ -783,7 +850,8 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------------------------------------------
 // Stage end from the accumulated right-hand side: U += J dt rhs, rho_k += J dt
 // rhs (div.jl:16-84), linComb with the stage-1 copies for stages 2 and 3
-// (utils.jl:95-112), c2Prim with the warm-started e -> T Newton (utils.jl:2-48).
+// (utils.jl:95-112), c2Prim with the warm-started e -> T Newton (utils.jl:2-48);
+// the pressure with the reciprocal molar masses (no division per species).
 // ---------------------------------------------------------------------------
 template <int NS>
 __global__ void __launch_bounds__(kBlock)
@@ -829,7 +897,7 @@ __global__ void __launch_bounds__(kBlock)
   const double ww = u[3] * rinv;
   const double ei = fmax(u[4] - 0.5 * rho * (uu * uu + vv * vv + ww * ww), kEps);
   const double T = fmax(get_T_from<NS>(P, ei, r, Q[p + 5 * NP]), kEps);
-  const double pr = fmax(pmixture<NS>(P, T, r), kEps);
+  const double pr = fmax(pmixture<NS, true>(P, T, r), kEps);
 #pragma unroll
   for (int n = 0; n < NS; ++n) rhoi[p + n * NP] = r[n];
   Q[p] = rho;
@@ -903,10 +971,10 @@ static void diff_ns(f3d_ctx* c, int dir, float* rhs, const double* Q, const floa
   if (dir == 0) {
     const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
     const long nb = (long)nseg * g.Ny * g.Nz;
-    hipLaunchKernelGGL((k_sw_diff_x<0, NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, Yi,
-                       M, J, lam, mu, D, c->dparams, g);
-    hipLaunchKernelGGL((k_sw_diff_x<1, NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, Yi,
-                       M, J, lam, mu, D, c->dparams, g);
+    hipLaunchKernelGGL(k_sw_visc_x, dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, M, J, lam, mu,
+                       g);
+    hipLaunchKernelGGL((k_sw_spec_x<NS>), dim3((unsigned)nb), dim3(kXSeg), 0, s, rhs, Q, Yi, M,
+                       J, D, c->dparams, g);
   } else {
     const int nd = dir == 1 ? g.Ny : g.Nz;
     const int nsv = (nd + kViscSeg - 1) / kViscSeg, sv = (nd + nsv - 1) / nsv;
