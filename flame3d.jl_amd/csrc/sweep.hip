@@ -497,7 +497,8 @@ __global__ void __launch_bounds__(kXSeg)
 // once: the metrics, J, rho and T of the cells -1 .. nc are staged in LDS,
 // then species by species Y_k of the cells -2 .. nc+1 and the transverse
 // differences and D_k of the cells -1 .. nc (two buffers, one barrier per
-// species); face f (between cells f-1 and f) forms its gradient from them
+// species, the next species' loads in flight across it); face f (between
+// cells f-1 and f) forms its gradient from them
 // (viscous.jl:430-507: the face average of the cells' transverse central
 // differences, the fourth-order normal difference).  A full segment has
 // nc + 1 = kXSeg + 1 faces: its last thread also takes the last face.
@@ -550,17 +551,41 @@ __global__ void __launch_bounds__(kXSeg)
     out[2] = (dY0 * m[2] + dY1 * m[5] + dY2 * m[8]) * Jac * rhoDi;
     out[3] = 0.5f * (sy[bb][f + 1] + sy[bb][f + 2]);
   };
+  // species n + 1's loads are issued before species n's barrier and faces
+  // (entries t and t + kXSeg of the staged ranges; the second exists for
+  // t < 4 in a full segment)
+  float ry[2], rt[2][5];
+  auto fetch = [&](int n) {
+    const float* y = Yi + (long)n * NP;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int i = t + k * kXSeg;
+      if (i < nc + 4) ry[k] = y[row + i - 2];
+      if (i < nc + 2) {
+        const long ic = row + i - 1;
+        rt[k][0] = y[ic + ty];
+        rt[k][1] = y[ic - ty];
+        rt[k][2] = y[ic + tz];
+        rt[k][3] = y[ic - tz];
+        rt[k][4] = D[ic + (long)n * NP];
+      }
+    }
+  };
+  fetch(0);
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
     const int bb = n & 1;
-    const float* y = Yi + (long)n * NP;
-    for (int i = t; i < nc + 4; i += kXSeg) sy[bb][i] = y[row + i - 2];
-    for (int i = t; i < nc + 2; i += kXSeg) {
-      const long ic = row + i - 1;
-      scy[bb][i] = y[ic + ty] - y[ic - ty];
-      scz[bb][i] = y[ic + tz] - y[ic - tz];
-      sd[bb][i] = D[ic + (long)n * NP];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int i = t + k * kXSeg;
+      if (i < nc + 4) sy[bb][i] = ry[k];
+      if (i < nc + 2) {
+        scy[bb][i] = rt[k][0] - rt[k][1];
+        scz[bb][i] = rt[k][2] - rt[k][3];
+        sd[bb][i] = rt[k][4];
+      }
     }
+    if (n + 1 < NS) fetch(n + 1);
     __syncthreads();
     float o[4];
     if (own) {
