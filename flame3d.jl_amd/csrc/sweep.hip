@@ -779,38 +779,71 @@ __global__ void __launch_bounds__(64)
   float y[NS][4];  // rows f-2 .. f+1
   float cX[NS][2], cT[NS][2], d2[NS][2];
   float mt[10][2], rho2[2], T2[2];
-  auto row_cross = [&](long ic, int r) {
+  // the loads of one step, fetched a step ahead: row f+1 of Y, the cross
+  // differences' inputs, D, metrics, J, rho and T of row f, and the
+  // right-hand-side entries of cell f-1
+  struct Row {
+    float y[NS], xp[NS], xm[NS], tp[NS], tm[NS], d[NS], mt[10], r[NS + 1];
+    double rho, T;
+  };
+  auto fetch = [&](int f, Row& R) {
+    const long ic = mm.base + (long)(f + NG) * mm.s;
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const float* yy = Yi + (long)n * NP;
-      cX[n][r] = yy[ic + 1] - yy[ic - 1];
-      cT[n][r] = yy[ic + t2] - yy[ic - t2];
-      d2[n][r] = D[ic + (long)n * NP];
+      R.y[n] = yy[ic + mm.s];
+      R.xp[n] = yy[ic + 1];
+      R.xm[n] = yy[ic - 1];
+      R.tp[n] = yy[ic + t2];
+      R.tm[n] = yy[ic - t2];
+      R.d[n] = D[ic + (long)n * NP];
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) mt[k][r] = M.m[k][ic];
-    mt[9][r] = J[ic];
-    rho2[r] = (float)Q[ic];
-    T2[r] = (float)Q[ic + 5 * NP];
+    for (int k = 0; k < 9; ++k) R.mt[k] = M.m[k][ic];
+    R.mt[9] = J[ic];
+    R.rho = Q[ic];
+    R.T = Q[ic + 5 * NP];
+    if (f > mm.c0) {
+      const long cell = mm.cell0 + (long)(f - 1) * mm.cstr;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) R.r[n] = rhs[(5 + n) * NC + cell];
+      R.r[NS] = rhs[4 * NC + cell];
+    }
   };
+  auto take_cross = [&](const Row& R) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      cX[n][1] = R.xp[n] - R.xm[n];
+      cT[n][1] = R.tp[n] - R.tm[n];
+      d2[n][1] = R.d[n];
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) mt[k][1] = R.mt[k];
+    rho2[1] = (float)R.rho;
+    T2[1] = (float)R.T;
+  };
+  Row nx;
   {
     const long ic0 = mm.base + (long)(mm.c0 + NG) * mm.s;
 #pragma unroll
     for (int n = 0; n < NS; ++n)
 #pragma unroll
       for (int r = 0; r < 3; ++r) y[n][r + 1] = Yi[(long)n * NP + ic0 + (r - 2) * mm.s];
-    row_cross(ic0 - mm.s, 1);
+    fetch(mm.c0 - 1, nx);  // row c0-1's cross inputs (its rhs entries are not read)
+    take_cross(nx);
+    fetch(mm.c0, nx);
   }
   const float c12 = 1.0f / 12.0f;
   float prev[NS + 1];
   for (int f = mm.c0; f <= mm.c1; ++f) {
-    const long ic = mm.base + (long)(f + NG) * mm.s;
+    const Row cu = nx;
+    if (f < mm.c1) fetch(f + 1, nx);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       y[n][0] = y[n][1];
       y[n][1] = y[n][2];
       y[n][2] = y[n][3];
-      y[n][3] = Yi[(long)n * NP + ic + mm.s];
+      y[n][3] = cu.y[n];
       cX[n][0] = cX[n][1];
       cT[n][0] = cT[n][1];
       d2[n][0] = d2[n][1];
@@ -819,7 +852,7 @@ __global__ void __launch_bounds__(64)
     for (int k = 0; k < 10; ++k) mt[k][0] = mt[k][1];
     rho2[0] = rho2[1];
     T2[0] = T2[1];
-    row_cross(ic, 1);
+    take_cross(cu);
     float m[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = (mt[k][0] + mt[k][1]) * 0.5f;
@@ -864,8 +897,8 @@ __global__ void __launch_bounds__(64)
     if (f > mm.c0) {
       const long cell = mm.cell0 + (long)(f - 1) * mm.cstr;
 #pragma unroll
-      for (int n = 0; n < NS; ++n) rhs_put(rhs, (5 + n) * NC + cell, cur[n] - prev[n], 1);
-      rhs_put(rhs, 4 * NC + cell, cur[NS] - prev[NS], 1);
+      for (int n = 0; n < NS; ++n) rhs[(5 + n) * NC + cell] = cu.r[n] + (cur[n] - prev[n]);
+      rhs[4 * NC + cell] = cu.r[NS] + (cur[NS] - prev[NS]);
     }
 #pragma unroll
     for (int n = 0; n <= NS; ++n) prev[n] = cur[n];
