@@ -398,74 +398,26 @@ struct FMet {
   const float* m[9];
 };
 
-__device__ inline void visc_face(const double* __restrict__ Q, const FMet& M,
-                                 const float* __restrict__ J, const float* __restrict__ lam,
-                                 const float* __restrict__ mu, const Geom& g, long ic, int dir,
-                                 float* out) {
-  const long st[3] = {1, (long)g.Nxt, (long)g.Nxt * g.Nyt};
-  const long s = st[dir];
-  const long im = ic - s;
-  const long NP = g.NP;
-  float m[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) m[q] = (M.m[q][im] + M.m[q][ic]) * 0.5f;
-  const float Jac = (J[im] + J[ic]) * 0.5f;
-  const float n0 = m[3 * dir], n1 = m[3 * dir + 1], n2 = m[3 * dir + 2];
-  const float c12 = 1.0f / 12.0f, c23 = 2.0f / 3.0f;
-  const float mui = (mu[im] + mu[ic]) * 0.5f;
-  const float lami = (lam[im] + lam[ic]) * 0.5f;
-  float dq[4][3];
-  const int qv[4] = {1, 2, 3, 5};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double* fq = Q + qv[q] * NP;
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      if (t == dir) {
-        dq[q][t] = 1.25f * ((float)fq[ic] - (float)fq[im]) -
-                   c12 * ((float)fq[ic + s] - (float)fq[ic - 2 * s]);
-      } else {
-        const long tt = st[t];
-        dq[q][t] = 0.5f * (c23 * ((float)fq[ic + tt] + (float)fq[im + tt] - (float)fq[ic - tt] -
-                                  (float)fq[im - tt]) -
-                           c12 * ((float)fq[ic + 2 * tt] + (float)fq[im + 2 * tt] -
-                                  (float)fq[ic - 2 * tt] - (float)fq[im - 2 * tt]));
-      }
-    }
-  }
-  const float u = ((float)Q[im + NP] + (float)Q[ic + NP]) * 0.5f;
-  const float v = ((float)Q[im + 2 * NP] + (float)Q[ic + 2 * NP]) * 0.5f;
-  const float w = ((float)Q[im + 3 * NP] + (float)Q[ic + 3 * NP]) * 0.5f;
-  float gr[4][3];
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      gr[q][j] = (dq[q][0] * m[j] + dq[q][1] * m[3 + j] + dq[q][2] * m[6 + j]) * Jac;
-  const float divu = gr[0][0] + gr[1][1] + gr[2][2];
-  const float t11 = mui * (2.0f * gr[0][0] - c23 * divu);
-  const float t12 = mui * (gr[0][1] + gr[1][0]);
-  const float t13 = mui * (gr[0][2] + gr[2][0]);
-  const float t22 = mui * (2.0f * gr[1][1] - c23 * divu);
-  const float t23 = mui * (gr[2][1] + gr[1][2]);
-  const float t33 = mui * (2.0f * gr[2][2] - c23 * divu);
-  const float E1 = u * t11 + v * t12 + w * t13 + lami * gr[3][0];
-  const float E2 = u * t12 + v * t22 + w * t23 + lami * gr[3][1];
-  const float E3 = u * t13 + v * t23 + w * t33 + lami * gr[3][2];
-  out[0] = n0 * t11 + n1 * t12 + n2 * t13;
-  out[1] = n0 * t12 + n1 * t22 + n2 * t23;
-  out[2] = n0 * t13 + n1 * t23 + n2 * t33;
-  out[3] = n0 * E1 + n1 * E2 + n2 * E3;
-}
-
-// x-line block of the viscous sweep: face values in LDS, then the
-// difference of each cell's two faces (rhs 1..4).
+// x-line block of the viscous sweep with each cell's inputs read once: the
+// metrics, J, mu and lambda of the cells -1 .. nc are staged in LDS, then
+// variable by variable (u, v, w, T) its values on the cells -2 .. nc+1 and
+// its transverse fourth-order differences (y, z) on the cells -1 .. nc (two
+// buffers, one barrier per variable, the next variable's loads in flight
+// across it); face f (between cells f-1 and f) takes its gradients from its
+// two cells (viscous.jl:1-142: the face average of the cells' transverse
+// differences, the fourth-order normal difference), then its stresses and
+// heat flux; the face values go back to LDS for the difference of each
+// cell's two faces (rhs 1..4).  A full segment has kXSeg + 1 faces: its last
+// thread also takes the last face.
 __global__ void __launch_bounds__(kXSeg)
     k_sw_visc_x(float* __restrict__ rhs, const double* __restrict__ Q, FMet M,
                 const float* __restrict__ J, const float* __restrict__ lam,
                 const float* __restrict__ mu, Geom g) {
-  constexpr int NO = 4;
-  __shared__ float sf[NO][kXSeg + 1];
+  constexpr int NE = kXSeg + 2;  // cells -1 .. nc
+  constexpr int NQ = kXSeg + 4;  // cells -2 .. nc + 1
+  __shared__ float sm[12][NE];   // metrics, J, mu, lambda
+  __shared__ float sq[2][NQ], sy[2][NE], sz[2][NE];
+  __shared__ float sf[4][kXSeg + 1];
   const int nseg = (g.Nx + kXSeg - 1) / kXSeg;
   const unsigned L = xcd_block();
   const int seg = (int)(L % (unsigned)nseg);
@@ -474,15 +426,99 @@ __global__ void __launch_bounds__(kXSeg)
   const int c0 = seg * kXSeg;
   const int nc = min(kXSeg, g.Nx - c0);
   const int t = threadIdx.x;
-  const long row = (long)(c0 + NG) + (long)g.Nxt * ((b + NG) + (long)g.Nyt * (c + NG));
-  auto face = [&](int f) {
-    float out[NO];
-    visc_face(Q, M, J, lam, mu, g, row + f, 0, out);
+  const long NP = g.NP, ty = g.Nxt, tz = (long)g.Nxt * g.Nyt;
+  const long row = (long)(c0 + NG) + ty * (b + NG) + tz * (c + NG);  // cell 0
+  const int qv[4] = {1, 2, 3, 5};  // u, v, w, T
+  // variable k's loads of entries t and t + kXSeg of the staged ranges (the
+  // second exists for t < 4 in a full segment)
+  double rq[2], rt[2][8];
+  auto fetch = [&](int k) {
+    const double* fq = Q + qv[k] * NP;
 #pragma unroll
-    for (int n = 0; n < NO; ++n) sf[n][f] = out[n];
+    for (int j = 0; j < 2; ++j) {
+      const int i = t + j * kXSeg;
+      if (i < nc + 4) rq[j] = fq[row + i - 2];
+      if (i < nc + 2) {
+        const long ic = row + i - 1;
+        rt[j][0] = fq[ic + ty];
+        rt[j][1] = fq[ic - ty];
+        rt[j][2] = fq[ic + 2 * ty];
+        rt[j][3] = fq[ic - 2 * ty];
+        rt[j][4] = fq[ic + tz];
+        rt[j][5] = fq[ic - tz];
+        rt[j][6] = fq[ic + 2 * tz];
+        rt[j][7] = fq[ic - 2 * tz];
+      }
+    }
   };
-  if (t < nc) face(t);            // the left face of cell c0 + t
-  if (t == nc - 1) face(nc);      // the segment's last face
+  fetch(0);
+  for (int i = t; i < nc + 2; i += kXSeg) {
+    const long ic = row + i - 1;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sm[q][i] = M.m[q][ic];
+    sm[9][i] = J[ic];
+    sm[10][i] = mu[ic];
+    sm[11][i] = lam[ic];
+  }
+  const float c12 = 1.0f / 12.0f, c23 = 2.0f / 3.0f;
+  const bool own = t <= nc;                          // face t
+  const bool extra = nc == kXSeg && t == kXSeg - 1;  // and face kXSeg
+  float dq[4][3], vel[3], dx[4][3], vx[3];           // face t's and the extra face's
+  auto grads = [&](int f, int bb, int k, float (&d)[4][3], float (&vv)[3]) {
+    d[k][0] = 1.25f * (sq[bb][f + 2] - sq[bb][f + 1]) - c12 * (sq[bb][f + 3] - sq[bb][f]);
+    d[k][1] = 0.5f * (sy[bb][f] + sy[bb][f + 1]);
+    d[k][2] = 0.5f * (sz[bb][f] + sz[bb][f + 1]);
+    if (k < 3) vv[k] = (sq[bb][f + 1] + sq[bb][f + 2]) * 0.5f;
+  };
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int bb = k & 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int i = t + j * kXSeg;
+      if (i < nc + 4) sq[bb][i] = (float)rq[j];
+      if (i < nc + 2) {
+        sy[bb][i] = c23 * ((float)rt[j][0] - (float)rt[j][1]) - c12 * ((float)rt[j][2] - (float)rt[j][3]);
+        sz[bb][i] = c23 * ((float)rt[j][4] - (float)rt[j][5]) - c12 * ((float)rt[j][6] - (float)rt[j][7]);
+      }
+    }
+    if (k + 1 < 4) fetch(k + 1);
+    __syncthreads();
+    if (own) grads(t, bb, k, dq, vel);
+    if (extra) grads(kXSeg, bb, k, dx, vx);
+  }
+  auto stress = [&](int f, const float (&d)[4][3], const float (&vv)[3]) {
+    float m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = (sm[q][f] + sm[q][f + 1]) * 0.5f;
+    const float Jac = (sm[9][f] + sm[9][f + 1]) * 0.5f;
+    const float mui = (sm[10][f] + sm[10][f + 1]) * 0.5f;
+    const float lami = (sm[11][f] + sm[11][f + 1]) * 0.5f;
+    float gr[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        gr[k][j] = (d[k][0] * m[j] + d[k][1] * m[3 + j] + d[k][2] * m[6 + j]) * Jac;
+    const float divu = gr[0][0] + gr[1][1] + gr[2][2];
+    const float t11 = mui * (2.0f * gr[0][0] - c23 * divu);
+    const float t12 = mui * (gr[0][1] + gr[1][0]);
+    const float t13 = mui * (gr[0][2] + gr[2][0]);
+    const float t22 = mui * (2.0f * gr[1][1] - c23 * divu);
+    const float t23 = mui * (gr[2][1] + gr[1][2]);
+    const float t33 = mui * (2.0f * gr[2][2] - c23 * divu);
+    const float u = vv[0], v = vv[1], w = vv[2];
+    const float E1 = u * t11 + v * t12 + w * t13 + lami * gr[3][0];
+    const float E2 = u * t12 + v * t22 + w * t23 + lami * gr[3][1];
+    const float E3 = u * t13 + v * t23 + w * t33 + lami * gr[3][2];
+    const float n0 = m[0], n1 = m[1], n2 = m[2];
+    sf[0][f] = n0 * t11 + n1 * t12 + n2 * t13;
+    sf[1][f] = n0 * t12 + n1 * t22 + n2 * t23;
+    sf[2][f] = n0 * t13 + n1 * t23 + n2 * t33;
+    sf[3][f] = n0 * E1 + n1 * E2 + n2 * E3;
+  };
+  if (own) stress(t, dq, vel);
+  if (extra) stress(kXSeg, dx, vx);
   __syncthreads();
   if (t < nc) {
     const long NC = (long)g.Nx * g.Ny * g.Nz;
