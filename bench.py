@@ -19,7 +19,7 @@ Both start from the same state; after the timed steps the FP32+NN fields are
 compared with the FP64-plain ones (the FP64 path matches the CPU oracle to
 1e-12 in tests/).  Cell-updates per RK stage = cells x 3 / time step.
 
-    python bench.py [--gpus 1] [--steps 6] [--warmup 2] [--size 256]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--size 256]
 """
 from __future__ import annotations
 
@@ -855,9 +855,9 @@ def bench_jet(a, torch, abi, cases, mm):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6,
+    ap.add_argument("--steps", type=int, default=20,
                     help="timed time steps (each = 3 RK3 stages + 2 chemistry half-steps)")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-size", type=int, default=48)
     ap.add_argument("--cpu-steps", type=int, default=2)
