@@ -145,7 +145,7 @@ def test_time_step_fp32_surrogate_vs_fp64_newton():
     """The benchmark's two paths on a 48x40x32 block of the reacting mixing
     layer over three time steps (six chemistry half-steps, nine RK3 stages),
     against FP64 plain (FP64 stage + FP64 Newton, which matches the oracle to
-    1e-12), at the north-star bars Y <= 1e-3 and T <= 1 K everywhere, for the
+    1e-12), at the accuracy bars Y <= 1e-3 and T <= 1 K everywhere, for the
     benchmarked path (FP32 stage + surrogate) and for each of its two parts:
     the surrogate with the FP64 stage, the FP32 stage with Newton."""
     size, steps = (48, 40, 32), 3

@@ -96,7 +96,7 @@ def test_jet_time_step_with_air_surrogate():
     stage + FP64 Newton air chemistry: FP32 stage + BF16 MLP air chemistry
     with gates and Newton fallback (the benchmarked form) and its two parts,
     the surrogate with the FP64 stage and the FP32 stage with Newton.  Bars
-    (north star): Y <= 1e-3 everywhere; T <= 1 K everywhere except next to a
+    (the accuracy target): Y <= 1e-3 everywhere; T <= 1 K everywhere except next to a
     face where the hybrid scheme took a different branch than in the
     reference run at some stage (within 2 cells).  The switch is
     discontinuous in the sensor, so any perturbation flips the faces whose
